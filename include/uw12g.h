@@ -1,0 +1,115 @@
+/* uw12g.h -- C ABI of the MI355X (gfx950) UW12 energy + Fock library.
+ *
+ * Drop-in for the libuw12 public API (zwill22/libuw12, libuw12.hpp:12-171):
+ * the same inputs (FP64, host, column-major, the reference's packing), the same
+ * validation, the same results; the work runs in HIP on one GPU.  Every entry
+ * point returns 0 on success or a negative status, never throws, and leaves a
+ * thread-local message for uw12g_last_error().
+ *
+ * Status codes: -1 invalid argument (the reference throws std::runtime_error /
+ * std::logic_error for the same inputs), -2 unsupported size, -3 HIP / device
+ * error, -4 numerical failure (eigensolver), -5 out of device memory.
+ *
+ * Layouts (as in the reference, libuw12.cpp:12-123):
+ *   X3     n_ao(n_ao+1)/2 x n_df, row = mu(mu+1)/2 + nu for nu <= mu
+ *   X2     n_df x n_df (symmetric to 1e-10)
+ *   X3_ri  (n_ao n_ri) x n_df, row = r + n_ri mu
+ *   S      (n_ao+n_ri)^2 overlap of the AO then RI functions
+ *   C      n_ao x (n_orb n_spin), alpha columns then beta; first n_occ are occupied
+ *   occ    n_occ[0] alpha values then n_occ[1] beta values, all > 0
+ *   fock   n_ao x (n_ao n_spin), alpha block then beta
+ */
+#ifndef UW12G_H
+#define UW12G_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct uw12g_context uw12g_context;
+typedef struct uw12g_base_integrals uw12g_base_integrals;
+typedef struct uw12g_abs_projectors uw12g_abs_projectors;
+
+/* Arithmetic of the bulk contractions.  FP64 is the plain form. */
+typedef enum { UW12G_FP64 = 0 } uw12g_precision;
+
+/* Term order of terms_out[] */
+enum {
+  UW12G_E2_DIRECT = 0,
+  UW12G_E2_INDIRECT = 1,
+  UW12G_E3_DIRECT = 2,
+  UW12G_E3_INDIRECT = 3,
+  UW12G_E4_DIRECT = 4,
+  UW12G_E4_INDIRECT = 5,
+  UW12G_N_TERMS = 6
+};
+
+/* Thread-local description of the last failure on this thread. */
+const char* uw12g_last_error(void);
+
+/* One HIP device + stream + device caches.  No reference counterpart: the
+ * reference keeps its state inside BaseIntegrals / Integrals objects. */
+int uw12g_context_create(int device, uw12g_context** out);
+void uw12g_context_destroy(uw12g_context* ctx);
+
+/* Replaces uw12::setup_base_integrals(X3, X2, X3_ri, n_ao, n_df, n_ri, copy)
+ * (libuw12.hpp:42-50, libuw12.cpp:22-36) and, with X3_ri == NULL and
+ * n_ri == 0, setup_base_integrals(X3, X2, n_ao, n_df, copy) (libuw12.hpp:64-70).
+ * Host arrays are copied to the device; the DF metric X2 is eigendecomposed in
+ * FP64 at creation (base_integrals.cpp:155-166). */
+int uw12g_setup_base_integrals(uw12g_context* ctx, const double* X3, const double* X2, const double* X3_ri,
+                               size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out);
+/* Same, for arrays that already live in device memory of the context's GPU. */
+int uw12g_setup_base_integrals_device(uw12g_context* ctx, const double* X3_dev, const double* X2,
+                                      const double* X3_ri_dev, size_t n_ao, size_t n_df, size_t n_ri,
+                                      uw12g_base_integrals** out);
+void uw12g_base_integrals_destroy(uw12g_base_integrals* ints);
+/* Number of DF metric eigenvectors kept (|lambda| >= 1e-10), or negative status. */
+long uw12g_base_integrals_kept(const uw12g_base_integrals* ints);
+
+/* Replaces uw12::setup_abs_projectors(S, n_ao, n_ri) (libuw12.hpp:80,
+ * ri_utils.hpp:45-74): eigenvalues |s| <= max(1e-6, 1e-8 max|s|) are dropped. */
+int uw12g_setup_abs_projectors(uw12g_context* ctx, const double* S, size_t n_ao, size_t n_ri,
+                               uw12g_abs_projectors** out);
+void uw12g_abs_projectors_destroy(uw12g_abs_projectors* abs);
+/* Copy the (n_ao+n_ri)^2 projector S^+ (AO first) to host memory. */
+int uw12g_abs_projectors_get(const uw12g_abs_projectors* abs, double* out);
+
+/* Replaces uw12::uw12_energy (libuw12.hpp:129-140, libuw12.cpp:156-182).
+ * orbitals/occupations are the arrays that setup_orbitals (libuw12.hpp:90-92)
+ * and setup_occupations (libuw12.hpp:101,111-113) wrap. */
+int uw12g_energy(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                 const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                 size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                 double scale_opp_spin, double scale_same_spin, double* energy);
+
+/* Replaces uw12::uw12_fock(double* fock, ...) (libuw12.hpp:157-169,
+ * libuw12.cpp:184-214). */
+int uw12g_fock(uw12g_context* ctx, double* fock, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+               const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+               size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+               double scale_opp_spin, double scale_same_spin, double* energy);
+
+/* Replaces uw12::form_fock (src/uw12.hpp:129-142, src/uw12.cpp:22-119): explicit
+ * indirect_term flag, optional Fock (fock == NULL: energy only) and the six
+ * energy pieces the reference prints (src/uw12.cpp:67-69,86-88,102-104). */
+int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                    const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                    size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                    int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
+                    double* fock, double* terms_out, double* energy);
+
+/* Kernel-level timing of the most recent evaluation on this context:
+ * names[i] / milliseconds[i] for up to cap entries; returns the count.
+ * Enabled by uw12g_context_set_profiling(ctx, 1).  No reference counterpart. */
+int uw12g_context_set_profiling(uw12g_context* ctx, int on);
+int uw12g_context_profile(const uw12g_context* ctx, const char** names, double* milliseconds,
+                          double* flops, double* bytes, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UW12G_H */

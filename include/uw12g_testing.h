@@ -1,0 +1,37 @@
+/* uw12g_testing.h -- test hooks of libuw12g (no reference counterpart).
+ *
+ * Exposes the strided FP64 GEMM building block so that its MFMA kernel can be
+ * checked in isolation against a one-thread-per-output kernel and the host.
+ * Every index of every operand uses the two-level map
+ *   off(i) = (i mod n0) s0 + (i div n0) s1   (n0 == 0: off(i) = i s0).
+ */
+#ifndef UW12G_TESTING_H
+#define UW12G_TESTING_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  long long s0, s1;
+  unsigned n0;
+} uw12g_map2;
+
+/* C <- alpha sum_k A(m,k) ks(k) B(k,n) + beta C for every batch, on device
+ * `device`; host arrays are copied in and C is copied back.  naive != 0 runs
+ * the reference kernel instead of the MFMA kernel; ks may be NULL. */
+int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
+                    size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, size_t nB,
+                    uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,
+                    uw12g_map2 Cn, long long Cb, const double* ks, size_t nks, uw12g_map2 Km, int naive);
+
+/* Host evaluation of the two-level map (checks the fast division). */
+long long uw12g_test_map2(uw12g_map2 m, unsigned i);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* UW12G_TESTING_H */

@@ -1,0 +1,61 @@
+"""Build libuw12g.so (HIP, gfx950) in-tree with hipcc.
+
+Usage: python -m libuw12_amd.build  (or libuw12_amd.build.build()).
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, 'csrc')
+OUT = os.path.join(HERE, 'libuw12g.so')
+BUILD_DIR = os.path.join(os.path.dirname(HERE), '_build', 'libuw12g')
+ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
+ARCH = os.environ.get('UW12G_ARCH', 'gfx950')
+
+SOURCES = ['gemm_f64.hip', 'kernels.hip', 'uw12g.cpp', 'testing.cpp']
+
+COMMON = [
+    '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
+    # keep hipcc from emitting packed FP32 instructions anywhere in this library
+    '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    '-Wall', '-Wno-unused-result', '-Wno-unused-variable',
+]
+
+
+def _hipcc():
+    p = os.path.join(ROCM, 'bin', 'hipcc')
+    return p if os.path.exists(p) else shutil.which('hipcc')
+
+
+def _compile(src):
+    obj = os.path.join(BUILD_DIR, os.path.basename(src) + '.o')
+    cmd = [_hipcc(), *COMMON, '-x', 'hip', '-c', os.path.join(CSRC, src), '-o', obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'hipcc failed for {src}:\n{r.stderr}')
+    return obj, r.stderr
+
+
+def build(verbose=False):
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(len(SOURCES), 8)) as ex:
+        results = list(ex.map(_compile, SOURCES))
+    if verbose:
+        for obj, err in results:
+            if err.strip():
+                print(err, file=sys.stderr)
+    tmp = OUT + '.tmp'
+    cmd = [_hipcc(), f'--offload-arch={ARCH}', '-shared', '-fPIC', '-o', tmp, *[o for o, _ in results],
+           f'-L{ROCM}/lib', '-lrocsolver', '-lrocblas', f'-Wl,-rpath,{ROCM}/lib']
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'link failed:\n{r.stderr}')
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == '__main__':
+    print(build(verbose=True))

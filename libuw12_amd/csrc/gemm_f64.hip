@@ -1,0 +1,342 @@
+// FP64 strided tensor-contraction GEMM on CDNA4 matrix cores.
+//
+// One kernel serves every contraction of the UW12 evaluation: each operand
+// index (rows, cols, batch) is addressed through a two-level map
+// off(i) = (i mod n0) s0 + (i div n0) s1, so sums over compound indices such
+// as (mu', A) or (j, k) run as a single K loop without materialising permuted
+// copies.  Tiles are staged through LDS; the inner product uses
+// v_mfma_f64_16x16x4_f64 with the B tile as the MFMA "A" operand so that lane
+// index maps to the output row m (column-major outputs store coalesced).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "uw12g_internal.h"
+
+namespace uw12g {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int BK = 16;
+constexpr int NT = 256;
+
+struct Params {
+  GemmDesc d;
+  int tiles_m, tiles_n, splits, kchunk;
+  double* out;  // split-K partials (dense, M x N per split and batch) or nullptr
+  int remap;    // 1: spread consecutive tiles over XCDs in contiguous chunks
+};
+
+template <int BM, int BN, bool AKC, bool BKC>
+__global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
+  constexpr int PAD = 4;
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int MB = WM / 16, NB = WN / 16;
+  constexpr int NA = BM * BK / NT;  // A elements loaded per thread
+  constexpr int NBL = BN * BK / NT;
+  __shared__ double As[BK][BM + PAD];
+  __shared__ double Bs[BK][BN + PAD];
+
+  const GemmDesc& d = p.d;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int n_tiles = p.tiles_m * p.tiles_n;
+  int tile = blockIdx.x;
+  if (p.remap) {  // blockIdx -> XCD is round-robin over 8 XCDs; give each XCD a contiguous range
+    const int per = n_tiles >> 3;
+    tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  }
+  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int b = blockIdx.z;
+  const int k_begin = blockIdx.y * p.kchunk;
+  const int k_end = min(d.K, k_begin + p.kchunk);
+
+  const double* Ap = d.A + d.Ab * b;
+  const double* Bp = d.B + d.Bb * b;
+  const double* Kp = d.ks ? d.ks + d.Kb * b : nullptr;
+
+  // ---- per-thread load assignment
+  // A: AKC -> kk fixed = t % BK, rows t/BK + (NT/BK) i ; else mm fixed = t % BM, kk = t/BM + (NT/BM) i
+  int64_t a_moff[AKC ? NA : 1];
+  bool a_mok[AKC ? NA : 1];
+  int a_kk0;
+  if constexpr (AKC) {
+    a_kk0 = t % BK;
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      const int m = m0 + t / BK + (NT / BK) * i;
+      a_mok[i] = m < d.M;
+      a_moff[i] = a_mok[i] ? d.Am.off(m) : 0;
+    }
+  } else {
+    a_kk0 = t / BM;
+    const int m = m0 + t % BM;
+    a_mok[0] = m < d.M;
+    a_moff[0] = a_mok[0] ? d.Am.off(m) : 0;
+  }
+  int64_t b_noff[BKC ? NBL : 1];
+  bool b_nok[BKC ? NBL : 1];
+  int b_kk0;
+  if constexpr (BKC) {
+    b_kk0 = t % BK;
+#pragma unroll
+    for (int i = 0; i < NBL; ++i) {
+      const int n = n0 + t / BK + (NT / BK) * i;
+      b_nok[i] = n < d.N;
+      b_noff[i] = b_nok[i] ? d.Bn.off(n) : 0;
+    }
+  } else {
+    b_kk0 = t / BN;
+    const int n = n0 + t % BN;
+    b_nok[0] = n < d.N;
+    b_noff[0] = b_nok[0] ? d.Bn.off(n) : 0;
+  }
+
+  double ra[NA], rb[NBL];
+  auto load = [&](int k0) {
+    if constexpr (AKC) {
+      const int k = k0 + a_kk0;
+      const bool kok = k < k_end;
+      const int64_t ko = kok ? d.Ak.off(k) : 0;
+      const double sc = (Kp && kok) ? Kp[d.Km.off(k)] : 1.0;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) ra[i] = (kok && a_mok[i]) ? Ap[a_moff[i] + ko] * sc : 0.0;
+    } else {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int k = k0 + a_kk0 + (NT / BM) * i;
+        const bool ok = k < k_end && a_mok[0];
+        double v = 0.0;
+        if (ok) {
+          v = Ap[a_moff[0] + d.Ak.off(k)];
+          if (Kp) v *= Kp[d.Km.off(k)];
+        }
+        ra[i] = v;
+      }
+    }
+    if constexpr (BKC) {
+      const int k = k0 + b_kk0;
+      const bool kok = k < k_end;
+      const int64_t ko = kok ? d.Bk.off(k) : 0;
+#pragma unroll
+      for (int i = 0; i < NBL; ++i) rb[i] = (kok && b_nok[i]) ? Bp[b_noff[i] + ko] : 0.0;
+    } else {
+#pragma unroll
+      for (int i = 0; i < NBL; ++i) {
+        const int k = k0 + b_kk0 + (NT / BN) * i;
+        const bool ok = k < k_end && b_nok[0];
+        rb[i] = ok ? Bp[b_noff[0] + d.Bk.off(k)] : 0.0;
+      }
+    }
+  };
+  auto store = [&]() {
+    if constexpr (AKC) {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) As[a_kk0][t / BK + (NT / BK) * i] = ra[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NA; ++i) As[a_kk0 + (NT / BM) * i][t % BM] = ra[i];
+    }
+    if constexpr (BKC) {
+#pragma unroll
+      for (int i = 0; i < NBL; ++i) Bs[b_kk0][t / BK + (NT / BK) * i] = rb[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NBL; ++i) Bs[b_kk0 + (NT / BN) * i][t % BN] = rb[i];
+    }
+  };
+
+  d4 acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  if (k_begin < k_end) {
+    load(k_begin);
+    for (int k0 = k_begin; k0 < k_end; k0 += BK) {
+      __syncthreads();
+      store();
+      __syncthreads();
+      if (k0 + BK < k_end) load(k0 + BK);
+#pragma unroll
+      for (int ks = 0; ks < BK; ks += 4) {
+        double af[MB], bf[NB];
+#pragma unroll
+        for (int i = 0; i < MB; ++i) af[i] = As[ks + lk][wm0 + 16 * i + lr];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) bf[j] = Bs[ks + lk][wn0 + 16 * j + lr];
+#pragma unroll
+        for (int i = 0; i < MB; ++i)
+#pragma unroll
+          for (int j = 0; j < NB; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+      }
+    }
+  }
+
+  // ---- epilogue: lane holds C[m = .. + lr][n = .. + 4*lk + r]
+  if (p.out) {
+    double* W = p.out + (int64_t(blockIdx.y) * d.batch + b) * int64_t(d.M) * d.N;
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      const int m = m0 + wm0 + 16 * i + lr;
+      if (m >= d.M) continue;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = n0 + wn0 + 16 * j + 4 * lk + r;
+          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * acc[i][j][r];
+        }
+    }
+    return;
+  }
+  double* Cp = d.C + d.Cb * b;
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    const int m = m0 + wm0 + 16 * i + lr;
+    if (m >= d.M) continue;
+    const int64_t mo = d.Cm.off(m);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = n0 + wn0 + 16 * j + 4 * lk + r;
+        if (n >= d.N) continue;
+        double* c = Cp + mo + d.Cn.off(n);
+        const double v = d.alpha * acc[i][j][r];
+        *c = d.beta == 0.0 ? v : v + d.beta * *c;
+      }
+  }
+}
+
+// Sum split-K partials and apply beta / the output map (fixed order: deterministic).
+__global__ void splitk_reduce_kernel(GemmDesc d, const double* W, int splits) {
+  const int64_t MN = int64_t(d.M) * d.N;
+  const int b = blockIdx.y;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < MN; e += int64_t(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int q = 0; q < splits; ++q) s += W[(int64_t(q) * d.batch + b) * MN + e];
+    const int m = int(e % d.M), n = int(e / d.M);
+    double* c = d.C + d.Cb * b + d.Cm.off(m) + d.Cn.off(n);
+    *c = d.beta == 0.0 ? s : s + d.beta * *c;
+  }
+}
+
+// Plain one-thread-per-output reference kernel (checks the MFMA kernel).
+__global__ void gemm_naive_kernel(GemmDesc d) {
+  const int64_t MN = int64_t(d.M) * d.N;
+  const int b = blockIdx.y;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < MN; e += int64_t(gridDim.x) * blockDim.x) {
+    const int m = int(e % d.M), n = int(e / d.M);
+    double s = 0.0;
+    for (int k = 0; k < d.K; ++k) {
+      double a = d.A[d.Ab * b + d.Am.off(m) + d.Ak.off(k)];
+      if (d.ks) a *= d.ks[d.Kb * b + d.Km.off(k)];
+      s += a * d.B[d.Bb * b + d.Bk.off(k) + d.Bn.off(n)];
+    }
+    double* c = d.C + d.Cb * b + d.Cm.off(m) + d.Cn.off(n);
+    *c = d.beta == 0.0 ? d.alpha * s : d.alpha * s + d.beta * *c;
+  }
+}
+
+template <int BM, int BN>
+void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
+  if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), grid, dim3(NT), 0, s, p);
+  else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), grid, dim3(NT), 0, s, p);
+  else if (bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), grid, dim3(NT), 0, s, p);
+}
+
+// Unit stride on the low level of a map (contiguous runs along that index).
+bool unit(const Map2& m, int extent) { return m.s0 == 1 && (m.n0 == 0 || m.n0 >= 2 || extent <= 1); }
+
+}  // namespace
+
+int64_t map_max(const Map2& m, int64_t extent) {
+  if (extent <= 0) return 0;
+  const int64_t last = extent - 1;
+  if (m.n0 == 0) return last * m.s0;
+  const int64_t q = last / m.n0, r = last % m.n0;
+  int64_t best = r * m.s0 + q * m.s1;
+  if (q >= 1) best = std::max(best, int64_t(m.n0 - 1) * m.s0 + (q - 1) * m.s1);
+  return best;
+}
+
+static void check_desc(const GemmDesc& d) {
+  if (d.K > 0) {
+    check_extent(d.A, d.Ab * (d.batch - 1) + map_max(d.Am, d.M) + map_max(d.Ak, d.K), "gemm A");
+    check_extent(d.B, d.Bb * (d.batch - 1) + map_max(d.Bk, d.K) + map_max(d.Bn, d.N), "gemm B");
+    if (d.ks) check_extent(d.ks, d.Kb * (d.batch - 1) + map_max(d.Km, d.K), "gemm kscale");
+  }
+  check_extent(d.C, d.Cb * (d.batch - 1) + map_max(d.Cm, d.M) + map_max(d.Cn, d.N), "gemm C");
+}
+
+void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  check_desc(d);
+  if (d.batch > 65535) throw Error(-2, "gemm batch exceeds 65535");
+  if (d.K <= 0) {  // C <- beta C
+    GemmDesc z = d;
+    z.K = 0;
+    gemm_f64_naive(z, stream);
+    return;
+  }
+  // Tile shape: large tiles for large outputs, small for thin ones.
+  int BM = d.M >= 96 ? 128 : 64;
+  int BN = d.N >= 96 ? 128 : 64;
+  const bool akc = unit(d.Ak, d.K) && !(unit(d.Am, d.M) && d.Am.n0 == 0 && d.Ak.s0 != 1);
+  const bool bkc = unit(d.Bk, d.K);
+  Params p;
+  p.d = d;
+  p.tiles_m = (d.M + BM - 1) / BM;
+  p.tiles_n = (d.N + BN - 1) / BN;
+  const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
+  if (tiles > 0x7fffffff) throw Error(-2, "gemm grid too large");
+  // Split K when the grid would leave most of the 256 CUs idle.
+  int splits = 1;
+  const int64_t wgs = tiles * d.batch;
+  const int kt = (d.K + BK - 1) / BK;
+  if (wgs < 512 && kt >= 16 && ws) {
+    splits = int(std::min<int64_t>((1024 + wgs - 1) / wgs, kt / 8));
+    splits = std::max(1, std::min(splits, 64));
+    while (splits > 1 && int64_t(splits) * d.batch > 65535) --splits;
+  }
+  p.kchunk = ((kt + splits - 1) / splits) * BK;
+  splits = (d.K + p.kchunk - 1) / p.kchunk;
+  p.splits = splits;
+  p.out = nullptr;
+  if (splits > 1) p.out = ws->get(size_t(splits) * d.batch * size_t(d.M) * d.N, stream);
+  p.remap = (tiles % 8 == 0 && tiles >= 64) ? 1 : 0;
+  dim3 grid(unsigned(tiles), unsigned(splits), unsigned(d.batch));
+  if (BM == 128 && BN == 128) launch_cfg<128, 128>(p, akc, bkc, grid, stream);
+  else if (BM == 128) launch_cfg<128, 64>(p, akc, bkc, grid, stream);
+  else if (BN == 128) launch_cfg<64, 128>(p, akc, bkc, grid, stream);
+  else launch_cfg<64, 64>(p, akc, bkc, grid, stream);
+  UW12G_HIP(hipGetLastError());
+  if (splits > 1) {
+    const int64_t MN = int64_t(d.M) * d.N;
+    dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
+    hipLaunchKernelGGL(splitk_reduce_kernel, g, dim3(256), 0, stream, d, p.out, splits);
+    UW12G_HIP(hipGetLastError());
+  }
+}
+
+void gemm_f64_naive(const GemmDesc& d, hipStream_t stream) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  check_desc(d);
+  const int64_t MN = int64_t(d.M) * d.N;
+  dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
+  hipLaunchKernelGGL(gemm_naive_kernel, g, dim3(256), 0, stream, d);
+  UW12G_HIP(hipGetLastError());
+}
+
+}  // namespace uw12g

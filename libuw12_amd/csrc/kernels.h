@@ -1,0 +1,42 @@
+// Host launchers for the bandwidth-bound / reduction kernels (kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace uw12g {
+
+constexpr int kMaxPartials = 4096;  // scratch doubles needed by every reduction
+
+// sum_i x[i] * (y ? y[i] : x[i]) * (row_w ? row_w[i % ld] : 1) * (col_w ? col_w[(i / ld) % col_n] : 1)
+struct DotDesc {
+  const double* x = nullptr;
+  const double* y = nullptr;
+  int64_t n = 0;
+  const double* row_w = nullptr;
+  const double* col_w = nullptr;
+  int64_t ld = 1;
+  int64_t col_n = 1;
+};
+
+void assemble_gamma(const double* X3p, const double* X3ri, double* G, int N, int Nr, int na, hipStream_t s);
+void gather_scale_cols(const double* Q, int n, const int* keep, const double* scale, double* T, int nk,
+                       hipStream_t s);
+void gemv_t(const double* G, int64_t rows, int64_t ldg, const double* x, int64_t ldx, int nvec, double* y,
+            int64_t ldy, int ncols, hipStream_t s);
+void gemv_n(const double* G, int64_t rows, int64_t ldg, const double* v, int64_t ldv, int nvec, double* y,
+            int64_t ldy, int ncols, hipStream_t s);
+// *out = (accumulate ? *out : 0) + scale * dot
+void dot(const DotDesc& d, double scale, double* out, int accumulate, double* partial, hipStream_t s);
+void e4i_energy(const double* W4, const double* V4, int a, int o, double scale, double* out, double* partial,
+                hipStream_t s);
+void e3i_energy(const double* GW, const double* GV, int M, int a, int nk, double scale, double* out,
+                double* partial, hipStream_t s);
+void combine_dd(int nW, int nV, const double* t0, const double* t1, const double* x0, const double* x1, double c0,
+                double c1, const double* sW, const double* sV, double* Tb, double* Z, hipStream_t s);
+void symmetrise_add(const double* F, double* out, int N, int accumulate, hipStream_t s);
+void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
+void fill(double* p, int64_t n, double v, hipStream_t s);
+
+}  // namespace uw12g

@@ -1,0 +1,287 @@
+// Bandwidth-bound and reduction kernels of the UW12 evaluation (FP64).
+// All reductions run in a fixed order (per-block partials, then one block),
+// so results are bit-reproducible run to run.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "kernels.h"
+#include "uw12g_internal.h"
+
+namespace uw12g {
+namespace {
+
+constexpr int RB = 256;  // reduction block size
+
+__device__ inline double block_sum(double v) {
+  __shared__ double sh[RB / 64];
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < RB / 64; ++i) r += sh[i];
+  __syncthreads();
+  return r;
+}
+
+// Gamma AO+RI assembly for a block of DF columns [a0, a0+na):
+// G[mu' + M alpha + M N j] = X3 (packed, mu' < N) or X3ri[r + Nr alpha] (mu' = N + r).
+__global__ void assemble_gamma_kernel(const double* __restrict__ X3p, const double* __restrict__ X3ri,
+                                      double* __restrict__ G, int N, int Nr, int na) {
+  const int M = N + Nr;
+  const int64_t per = int64_t(M) * N;
+  const int64_t total = per * na;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int j = int(e / per);
+    const int64_t rem = e - int64_t(j) * per;
+    const int alpha = int(rem / M), mu = int(rem - int64_t(alpha) * M);
+    double v;
+    if (mu < N) {
+      const int hi = max(mu, alpha), lo = min(mu, alpha);
+      v = X3p[int64_t(hi) * (hi + 1) / 2 + lo + int64_t(N) * (N + 1) / 2 * j];
+    } else {
+      v = X3ri[(mu - N) + int64_t(Nr) * alpha + int64_t(Nr) * N * j];
+    }
+    G[e] = v;
+  }
+}
+
+// T[:, j] = Q[:, keep[j]] * scale[j]
+__global__ void gather_scale_cols_kernel(const double* Q, int n, const int* keep, const double* scale,
+                                         double* T, int nk) {
+  const int64_t total = int64_t(n) * nk;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int j = int(e / n), i = int(e - int64_t(j) * n);
+    T[e] = Q[i + int64_t(n) * keep[j]] * scale[j];
+  }
+}
+
+// y[c][A] = sum_r G[r + ldg A] x[c][r] for nvec vectors; one block per column A.
+__global__ void gemv_t_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg, const double* __restrict__ x,
+                              int64_t ldx, int nvec, double* __restrict__ y, int64_t ldy, int ncols) {
+  const int A = blockIdx.x;
+  if (A >= ncols) return;
+  const double* g = G + ldg * A;
+  double acc[2] = {0.0, 0.0};
+  for (int64_t r = threadIdx.x; r < rows; r += RB) {
+    const double gv = g[r];
+    acc[0] += gv * x[r];
+    if (nvec > 1) acc[1] += gv * x[ldx + r];
+  }
+  for (int c = 0; c < nvec; ++c) {
+    const double s = block_sum(acc[c]);
+    if (threadIdx.x == 0) y[ldy * c + A] = s;
+  }
+}
+
+// y[c][r] += sum_A G[r + ldg A] v[c][A]  (rows split across threads, A loop in order)
+__global__ void gemv_n_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg, const double* __restrict__ v,
+                              int64_t ldv, int nvec, double* __restrict__ y, int64_t ldy, int ncols) {
+  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < rows; r += int64_t(gridDim.x) * blockDim.x) {
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int A = 0; A < ncols; ++A) {
+      const double gv = G[r + ldg * A];
+      acc0 += gv * v[A];
+      if (nvec > 1) acc1 += gv * v[ldv + A];
+    }
+    y[r] += acc0;
+    if (nvec > 1) y[ldy + r] += acc1;
+  }
+}
+
+// partial[b] = sum over this block's share of  x[i] * y[i] * w(i)
+//   w(i) = (ws ? ws[(i / wdiv) % wmod] : 1) * (vs ? vs[(i % vmodr) ...]) -- see dot_desc
+__global__ void dot_kernel(DotDesc d, double* partial) {
+  double acc = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(RB) + threadIdx.x; i < d.n; i += int64_t(gridDim.x) * RB) {
+    double v = d.x[i] * (d.y ? d.y[i] : d.x[i]);
+    if (d.row_w) v *= d.row_w[i % d.ld];
+    if (d.col_w) v *= d.col_w[(i / d.ld) % d.col_n];
+    acc += v;
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ void finish_sum_kernel(const double* partial, int n, double scale, double* out, int accumulate) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += RB) acc += partial[i];
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) *out = accumulate ? *out + scale * s : scale * s;
+}
+
+// 4e indirect: sum_{i,j<a; k,l<o} W4[(i + a l) + ao (j + a k)] V4[(i + a k) + ao (j + a l)]
+__global__ void e4i_kernel(const double* __restrict__ W4, const double* __restrict__ V4, int a, int o,
+                           double* partial) {
+  const int64_t ao = int64_t(a) * o;
+  const int64_t total = ao * ao;
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * int64_t(RB) + threadIdx.x; e < total; e += int64_t(gridDim.x) * RB) {
+    const int64_t row = e % ao, col = e / ao;  // V4 element (i + a k, j + a l)
+    const int i = int(row % a), k = int(row / a);
+    const int j = int(col % a), l = int(col / a);
+    acc += W4[(i + int64_t(a) * l) + ao * (j + int64_t(a) * k)] * V4[e];
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// 3e indirect energy: sum GW[mu + M i + M a (j + a kk)] GV[mu + M j + M a (i + a kk)]
+// over mu < M, i, j < a and kk in [0, nk) (a k-block of the occupied index).
+__global__ void e3i_kernel(const double* __restrict__ GW, const double* __restrict__ GV, int M, int a, int nk,
+                           double* partial) {
+  const int64_t Ma = int64_t(M) * a;
+  const int64_t total = Ma * a * nk;
+  double acc = 0.0;
+  for (int64_t e = blockIdx.x * int64_t(RB) + threadIdx.x; e < total; e += int64_t(gridDim.x) * RB) {
+    const int64_t r = e % Ma, c = e / Ma;  // GW row (mu, i), col (j, kk)
+    const int mu = int(r % M), i = int(r / M);
+    const int j = int(c % a), kk = int(c / a);
+    acc += GW[e] * GV[mu + int64_t(M) * j + Ma * (i + int64_t(a) * kk)];
+  }
+  const double s = block_sum(acc);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Direct-term matrices on the DF x DF grid:
+//   Tb = sW sV sum_u c_u t_u ;  Z = Tb - sW sV sum_u c_u X_u
+__global__ void combine_dd_kernel(int nW, int nV, const double* t0, const double* t1, const double* x0,
+                                  const double* x1, double c0, double c1, const double* sW, const double* sV,
+                                  double* Tb, double* Z) {
+  const int64_t total = int64_t(nW) * nV;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int A = int(e % nW), B = int(e / nW);
+    const double sg = sW[A] * sV[B];
+    double tb = c0 * t0[e];
+    double xb = c0 * x0[e];
+    if (t1) {
+      tb += c1 * t1[e];
+      xb += c1 * x1[e];
+    }
+    tb *= sg;
+    Tb[e] = tb;
+    Z[e] = tb - sg * xb;
+  }
+}
+
+// F_out[s] (+)= 0.5 (F + F^T)
+__global__ void symmetrise_add_kernel(const double* F, double* out, int N, int accumulate) {
+  const int64_t total = int64_t(N) * N;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int i = int(e % N), j = int(e / N);
+    const double v = 0.5 * (F[e] + F[j + int64_t(N) * i]);
+    out[e] = accumulate ? out[e] + v : v;
+  }
+}
+
+// out[r + rows c] = in[r + rows c] * sc[c]
+__global__ void scale_cols_kernel(const double* __restrict__ in, int64_t rows, int64_t cols,
+                                  const double* __restrict__ sc, double* __restrict__ out) {
+  const int64_t total = rows * cols;
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x)
+    out[e] = in[e] * sc[e / rows];
+}
+
+__global__ void fill_kernel(double* p, int64_t n, double v) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x)
+    p[e] = v;
+}
+
+unsigned grid_for(int64_t n, int64_t cap = 8192) {
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + 255) / 256, cap)));
+}
+
+}  // namespace
+
+void assemble_gamma(const double* X3p, const double* X3ri, double* G, int N, int Nr, int na, hipStream_t s) {
+  const int64_t total = int64_t(N + Nr) * N * na;
+  hipLaunchKernelGGL(assemble_gamma_kernel, dim3(grid_for(total, 65536)), dim3(256), 0, s, X3p, X3ri, G, N, Nr, na);
+  UW12G_HIP(hipGetLastError());
+}
+
+void gather_scale_cols(const double* Q, int n, const int* keep, const double* scale, double* T, int nk,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(gather_scale_cols_kernel, dim3(grid_for(int64_t(n) * nk)), dim3(256), 0, s, Q, n, keep, scale,
+                     T, nk);
+  UW12G_HIP(hipGetLastError());
+}
+
+void gemv_t(const double* G, int64_t rows, int64_t ldg, const double* x, int64_t ldx, int nvec, double* y,
+            int64_t ldy, int ncols, hipStream_t s) {
+  if (ncols <= 0) return;
+  if (nvec > 2) throw Error(-2, "gemv_t supports at most 2 vectors");
+  hipLaunchKernelGGL(gemv_t_kernel, dim3(ncols), dim3(RB), 0, s, G, rows, ldg, x, ldx, nvec, y, ldy, ncols);
+  UW12G_HIP(hipGetLastError());
+}
+
+void gemv_n(const double* G, int64_t rows, int64_t ldg, const double* v, int64_t ldv, int nvec, double* y,
+            int64_t ldy, int ncols, hipStream_t s) {
+  if (rows <= 0) return;
+  if (nvec > 2) throw Error(-2, "gemv_n supports at most 2 vectors");
+  hipLaunchKernelGGL(gemv_n_kernel, dim3(grid_for(rows, 1 << 20)), dim3(256), 0, s, G, rows, ldg, v, ldv, nvec, y,
+                     ldy, ncols);
+  UW12G_HIP(hipGetLastError());
+}
+
+void dot(const DotDesc& d, double scale, double* out, int accumulate, double* partial, hipStream_t s) {
+  const int nb = int(std::min<int64_t>((d.n + RB - 1) / RB, kMaxPartials));
+  if (d.n <= 0) {
+    hipLaunchKernelGGL(fill_kernel, dim3(1), dim3(64), 0, s, partial, int64_t(1), 0.0);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(RB), 0, s, partial, 1, scale, out, accumulate);
+  } else {
+    hipLaunchKernelGGL(dot_kernel, dim3(nb), dim3(RB), 0, s, d, partial);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(RB), 0, s, partial, nb, scale, out, accumulate);
+  }
+  UW12G_HIP(hipGetLastError());
+}
+
+void e4i_energy(const double* W4, const double* V4, int a, int o, double scale, double* out, double* partial,
+                hipStream_t s) {
+  const int64_t total = int64_t(a) * o * a * o;
+  const int nb = int(std::max<int64_t>(1, std::min<int64_t>((total + RB - 1) / RB, kMaxPartials)));
+  hipLaunchKernelGGL(e4i_kernel, dim3(nb), dim3(RB), 0, s, W4, V4, a, o, partial);
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(RB), 0, s, partial, nb, scale, out, 1);
+  UW12G_HIP(hipGetLastError());
+}
+
+void e3i_energy(const double* GW, const double* GV, int M, int a, int nk, double scale, double* out,
+                double* partial, hipStream_t s) {
+  const int64_t total = int64_t(M) * a * a * nk;
+  const int nb = int(std::max<int64_t>(1, std::min<int64_t>((total + RB - 1) / RB, kMaxPartials)));
+  hipLaunchKernelGGL(e3i_kernel, dim3(nb), dim3(RB), 0, s, GW, GV, M, a, nk, partial);
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(RB), 0, s, partial, nb, scale, out, 1);
+  UW12G_HIP(hipGetLastError());
+}
+
+void combine_dd(int nW, int nV, const double* t0, const double* t1, const double* x0, const double* x1, double c0,
+                double c1, const double* sW, const double* sV, double* Tb, double* Z, hipStream_t s) {
+  hipLaunchKernelGGL(combine_dd_kernel, dim3(grid_for(int64_t(nW) * nV)), dim3(256), 0, s, nW, nV, t0, t1, x0, x1,
+                     c0, c1, sW, sV, Tb, Z);
+  UW12G_HIP(hipGetLastError());
+}
+
+void symmetrise_add(const double* F, double* out, int N, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(symmetrise_add_kernel, dim3(grid_for(int64_t(N) * N)), dim3(256), 0, s, F, out, N, accumulate);
+  UW12G_HIP(hipGetLastError());
+}
+
+void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, in, rows, cols, sc, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void fill(double* p, int64_t n, double v, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+  UW12G_HIP(hipGetLastError());
+}
+
+}  // namespace uw12g

@@ -1,0 +1,82 @@
+// Test hooks (include/uw12g_testing.h).
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/uw12g_testing.h"
+#include "uw12g_internal.h"
+
+namespace uw12g {
+void set_last_error(const std::string& msg);
+}
+
+using namespace uw12g;
+
+namespace {
+Map2 to_map(const uw12g_map2& m) { return m.n0 == 0 ? lin(m.s0) : two(m.n0, m.s0, m.s1); }
+
+struct Dev {
+  double* p = nullptr;
+  ~Dev() {
+    if (p) {
+      unregister_alloc(p);
+      (void)hipFree(p);
+    }
+  }
+  void up(const double* h, size_t n) {
+    UW12G_HIP(hipMalloc(&p, sizeof(double) * (n ? n : 1)));
+    register_alloc(p, sizeof(double) * (n ? n : 1));
+    if (h && n) UW12G_HIP(hipMemcpy(p, h, sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+};
+}  // namespace
+
+extern "C" long long uw12g_test_map2(uw12g_map2 m, unsigned i) { return (long long)to_map(m).off(i); }
+
+extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, double beta,
+                               const double* A, size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab,
+                               const double* B, size_t nB, uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C,
+                               size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
+                               uw12g_map2 Km, int naive) {
+  try {
+    UW12G_HIP(hipSetDevice(device));
+    Dev a, b, c, k;
+    a.up(A, nA);
+    b.up(B, nB);
+    c.up(C, nC);
+    if (ks) k.up(ks, nks);
+    GemmDesc d;
+    d.M = M;
+    d.N = N;
+    d.K = K;
+    d.batch = batch;
+    d.alpha = alpha;
+    d.beta = beta;
+    d.A = a.p;
+    d.Am = to_map(Am);
+    d.Ak = to_map(Ak);
+    d.Ab = Ab;
+    d.B = b.p;
+    d.Bk = to_map(Bk);
+    d.Bn = to_map(Bn);
+    d.Bb = Bb;
+    d.C = c.p;
+    d.Cm = to_map(Cm);
+    d.Cn = to_map(Cn);
+    d.Cb = Cb;
+    d.ks = ks ? k.p : nullptr;
+    d.Km = to_map(Km);
+    Workspace ws;
+    if (naive) gemm_f64_naive(d, nullptr);
+    else gemm_f64(d, nullptr, &ws);
+    UW12G_HIP(hipDeviceSynchronize());
+    UW12G_HIP(hipMemcpy(C, c.p, sizeof(double) * nC, hipMemcpyDeviceToHost));
+    return 0;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return -1;
+  }
+}

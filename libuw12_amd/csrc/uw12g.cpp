@@ -1,0 +1,1327 @@
+// libuw12g: C ABI + host orchestration of the UW12 energy / Fock evaluation on
+// one MI355X.  See include/uw12g.h for the contract.
+//
+// Data layout in HBM (per base-integrals object X in {W, V, WV}):
+//   Gamma_X[mu' + M alpha + M N A]  (M = n_ao + n_ri, N = n_ao, A < D_X)
+//     = sum_B G0_X[mu', alpha, B] Q_X[B, A] / sqrt|lambda_A|,  s_X[A] = sign(lambda_A)
+//   with G0[mu' < N] the unpacked AO block of X3 and G0[N + r] = X3_ri[r + n_ri alpha].
+//   Every DF-metric contraction of the reference, sum_A (..|A~) lambda_A (A~|..),
+//   equals sum_A (..|Gamma_A) s_A (Gamma_A|..) for any factorisation of X2
+//   (base_integrals.cpp:155-166 uses P2 = Q pinv(Lambda)); the balanced form keeps
+//   |Gamma| bounded where 1/lambda would not be.
+// Per evaluation and spin s (o occupied, a active, c = o - a core):
+//   Phi_X[mu', k, A]   = sum_alpha Gamma_X[mu', alpha, A] Co[alpha, k]    (mu' k | X | A)
+//   O2_X[i, k, A]      = sum_mu Ca[mu, i] Phi_X[mu, k, A]                  (i k | X | A)
+//   PhiT_V[mu', i, B]  = sum_nu' S+[mu', nu'] Phi_V[nu', c+i, B]
+// and every energy / Fock term is a strided GEMM over these (gemm_f64.hip).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/uw12g.h"
+#include "kernels.h"
+#include "uw12g_internal.h"
+
+namespace uw12g {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+std::mutex g_alloc_mu;
+std::map<uintptr_t, size_t>& allocs() {
+  static std::map<uintptr_t, size_t> m;
+  return m;
+}
+}  // namespace
+
+void register_alloc(const void* p, size_t bytes) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  allocs()[reinterpret_cast<uintptr_t>(p)] = bytes;
+}
+
+void unregister_alloc(const void* p) {
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  allocs().erase(reinterpret_cast<uintptr_t>(p));
+}
+
+void check_extent(const void* base, int64_t max_off, const char* what) {
+  if (!base) throw Error(-2, std::string(what) + ": null operand");
+  if (max_off < 0) throw Error(-2, std::string(what) + ": negative extent");
+  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  const uintptr_t b = reinterpret_cast<uintptr_t>(base);
+  auto& m = allocs();
+  auto it = m.upper_bound(b);
+  if (it == m.begin()) return;  // not library-owned
+  --it;
+  if (b >= it->first + it->second) return;  // not inside a library allocation
+  const uintptr_t end = b + uintptr_t(max_off + 1) * sizeof(double);
+  if (end > it->first + it->second)
+    throw Error(-2, std::string(what) + ": operand extent exceeds its allocation (" +
+                        std::to_string(end - it->first) + " > " + std::to_string(it->second) + " bytes)");
+}
+
+double* Workspace::get(size_t n, hipStream_t stream) {
+  const size_t need = std::max<size_t>(n, 1) * sizeof(double);
+  if (need > bytes) {
+    if (ptr) {
+      UW12G_HIP(hipStreamSynchronize(stream));
+      unregister_alloc(ptr);
+      (void)hipFree(ptr);
+    }
+    ptr = nullptr;
+    bytes = 0;
+    if (hipMalloc(&ptr, need) != hipSuccess) throw Error(-5, "out of device memory (workspace)");
+    register_alloc(ptr, need);
+    bytes = need;
+  }
+  return static_cast<double*>(ptr);
+}
+
+Workspace::~Workspace() {
+  if (ptr) {
+    unregister_alloc(ptr);
+    (void)hipFree(ptr);
+  }
+}
+
+// Device buffer owned by RAII.
+struct DBuf {
+  double* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { reset(); }
+  void reset() {
+    if (p) {
+      unregister_alloc(p);
+      (void)hipFree(p);
+    }
+    p = nullptr;
+    n = 0;
+  }
+  double* get(size_t count) {
+    if (count > n || !p) {
+      reset();
+      if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)) != hipSuccess)
+        throw Error(-5, "out of device memory (" + std::to_string(count * 8.0 / 1e9) + " GB requested)");
+      n = std::max<size_t>(count, 1);
+      register_alloc(p, n * sizeof(double));
+    }
+    return p;
+  }
+};
+
+struct ProfileRec {
+  std::string name;
+  hipEvent_t start, stop;
+  double flops, bytes;
+};
+
+}  // namespace uw12g
+
+using namespace uw12g;
+
+struct uw12g_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle rb = nullptr;
+  Workspace gemm_ws;
+  std::map<std::string, std::unique_ptr<DBuf>> bufs;
+  bool profiling = false;
+  std::vector<ProfileRec> prof;
+  std::vector<std::string> prof_names;  // stable storage for names handed out
+
+  // Named device buffers reused across evaluations; growing or releasing one
+  // first drains the stream so no queued kernel still reads the old memory.
+  double* buf(const std::string& name, size_t n) {
+    auto& b = bufs[name];
+    if (!b) b.reset(new DBuf());
+    if (b->p && n > b->n) UW12G_HIP(hipStreamSynchronize(stream));
+    return b->get(n);
+  }
+  void release(const std::string& name) {
+    auto it = bufs.find(name);
+    if (it == bufs.end()) return;
+    UW12G_HIP(hipStreamSynchronize(stream));
+    bufs.erase(it);
+  }
+  ~uw12g_context() {
+    for (auto& r : prof) {
+      (void)hipEventDestroy(r.start);
+      (void)hipEventDestroy(r.stop);
+    }
+    bufs.clear();
+    if (rb) rocblas_destroy_handle(rb);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+struct uw12g_base_integrals {
+  uw12g_context* ctx = nullptr;
+  int N = 0, Nd = 0, Nr = 0, M = 0, D = 0;
+  DBuf gamma;  // M N D
+  DBuf sign;   // D
+  std::vector<double> sign_host;
+};
+
+struct uw12g_abs_projectors {
+  uw12g_context* ctx = nullptr;
+  int N = 0, Nr = 0, M = 0;
+  DBuf Sp;  // M x M
+};
+
+namespace {
+
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return -1;
+  } catch (...) {
+    g_last_error = "unknown error";
+    return -1;
+  }
+}
+
+void require(bool ok, const std::string& msg, int code = -1) {
+  if (!ok) throw Error(code, msg);
+}
+
+void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)); }
+
+// GEMM launch with optional per-launch timing.
+void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* name) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  ProfileRec rec;
+  if (ctx->profiling) {
+    UW12G_HIP(hipEventCreate(&rec.start));
+    UW12G_HIP(hipEventCreate(&rec.stop));
+    UW12G_HIP(hipEventRecord(rec.start, ctx->stream));
+  }
+  gemm_f64(d, ctx->stream, &ctx->gemm_ws);
+  if (ctx->profiling) {
+    UW12G_HIP(hipEventRecord(rec.stop, ctx->stream));
+    rec.name = name;
+    rec.flops = 2.0 * d.M * double(d.N) * d.K * d.batch;
+    rec.bytes = 0;
+    ctx->prof.push_back(rec);
+  }
+}
+
+template <class F>
+void timed(uw12g_context* ctx, const char* name, double flops, double bytes, F&& f) {
+  if (!ctx->profiling) {
+    f();
+    return;
+  }
+  ProfileRec rec;
+  UW12G_HIP(hipEventCreate(&rec.start));
+  UW12G_HIP(hipEventCreate(&rec.stop));
+  UW12G_HIP(hipEventRecord(rec.start, ctx->stream));
+  f();
+  UW12G_HIP(hipEventRecord(rec.stop, ctx->stream));
+  rec.name = name;
+  rec.flops = flops;
+  rec.bytes = bytes;
+  ctx->prof.push_back(rec);
+}
+
+// FP64 symmetric eigendecomposition on the device (rocSOLVER syevd, lower
+// triangle, ascending eigenvalues -- the LAPACK routine behind Armadillo's
+// eig_sym used at linalg.hpp:1112-1145).  A (device, n x n) is overwritten by
+// the eigenvectors; eigenvalues are returned on the host.
+std::vector<double> eig_device(uw12g_context* ctx, double* A, int n) {
+  std::vector<double> w(n);
+  if (n == 0) return w;
+  DBuf D, E;
+  D.get(n);
+  E.get(n);
+  int* info = nullptr;
+  UW12G_HIP(hipMalloc(&info, sizeof(int)));
+  const rocblas_status st =
+      rocsolver_dsyevd(ctx->rb, rocblas_evect_original, rocblas_fill_lower, n, A, n, D.p, E.p, info);
+  int hinfo = 0;
+  UW12G_HIP(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  UW12G_HIP(hipMemcpyAsync(w.data(), D.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+  UW12G_HIP(hipStreamSynchronize(ctx->stream));
+  (void)hipFree(info);
+  if (st != rocblas_status_success) throw Error(-4, "rocsolver_dsyevd failed (status " + std::to_string(st) + ")");
+  if (hinfo != 0) throw Error(-4, "eigen-decomposition failed");
+  return w;
+}
+
+bool is_symmetric(const double* X, size_t n, double tol) {
+  for (size_t j = 0; j < n; ++j)
+    for (size_t i = j + 1; i < n; ++i)
+      if (std::fabs(X[i + n * j] - X[j + n * i]) > tol) return false;
+  return true;
+}
+
+void setup_base(uw12g_context* ctx, const double* X3, const double* X2, const double* X3ri, size_t n_ao,
+                size_t n_df, size_t n_ri, bool device_ptrs, uw12g_base_integrals** out) {
+  require(ctx != nullptr, "null context");
+  require(out != nullptr, "null output handle");
+  require(X3 != nullptr && X2 != nullptr, "null integral array");
+  require(n_ao > 0, "Number of ao orbitals is zero");
+  require(n_df > 0, "Number of density-fitting basis functions is zero");
+  require((X3ri == nullptr) == (n_ri == 0), "X3_ri and n_ri must both be given or both be absent");
+  require(n_ao < 46341 && n_df < (1u << 30) && n_ri < (1u << 30), "size out of range", -2);
+  require(is_symmetric(X2, n_df, 1e-10), "matrix must be symmetric");
+  set_device(ctx);
+  std::unique_ptr<uw12g_base_integrals> I(new uw12g_base_integrals());
+  I->ctx = ctx;
+  const int N = int(n_ao), Nd = int(n_df), Nr = int(n_ri), M = N + Nr;
+  I->N = N;
+  I->Nd = Nd;
+  I->Nr = Nr;
+  I->M = M;
+  hipStream_t s = ctx->stream;
+  // ---- DF metric: eig on device; keep |lambda| >= 1e-10 (pinv tolerance, linalg.hpp:469-481)
+  DBuf Q;
+  Q.get(size_t(Nd) * Nd);
+  UW12G_HIP(hipMemcpyAsync(Q.p, X2, sizeof(double) * Nd * Nd, hipMemcpyHostToDevice, s));
+  std::vector<double> lam = eig_device(ctx, Q.p, Nd);
+  std::vector<int> keep;
+  std::vector<double> scale;
+  for (int A = 0; A < Nd; ++A)
+    if (std::fabs(lam[A]) >= 1e-10) {
+      keep.push_back(A);
+      scale.push_back(1.0 / std::sqrt(std::fabs(lam[A])));
+      I->sign_host.push_back(lam[A] > 0 ? 1.0 : -1.0);
+    }
+  const int D = int(keep.size());
+  I->D = D;
+  DBuf T;
+  T.get(size_t(Nd) * std::max(D, 1));
+  I->sign.get(std::max(D, 1));
+  if (D > 0) {
+    int* keep_d = nullptr;
+    DBuf sc;
+    sc.get(D);
+    UW12G_HIP(hipMalloc(&keep_d, sizeof(int) * D));
+    UW12G_HIP(hipMemcpyAsync(keep_d, keep.data(), sizeof(int) * D, hipMemcpyHostToDevice, s));
+    UW12G_HIP(hipMemcpyAsync(sc.p, scale.data(), sizeof(double) * D, hipMemcpyHostToDevice, s));
+    UW12G_HIP(hipMemcpyAsync(I->sign.p, I->sign_host.data(), sizeof(double) * D, hipMemcpyHostToDevice, s));
+    gather_scale_cols(Q.p, Nd, keep_d, sc.p, T.p, D, s);
+    UW12G_HIP(hipStreamSynchronize(s));
+    (void)hipFree(keep_d);
+  }
+  Q.reset();
+  // ---- Gamma = G0 T, streamed over DF column blocks of the input
+  const size_t MN = size_t(M) * N;
+  I->gamma.get(MN * std::max(D, 1));
+  if (D == 0) {
+    *out = I.release();
+    return;
+  }
+  const size_t P = size_t(N) * (N + 1) / 2;
+  const size_t budget = size_t(1) << 28;  // doubles per staging block (2 GiB)
+  int cA = int(std::max<size_t>(1, std::min<size_t>(Nd, budget / std::max<size_t>(MN, 1))));
+  DBuf G0, st3, stri;
+  G0.get(MN * cA);
+  if (!device_ptrs) {
+    st3.get(P * cA);
+    if (Nr > 0) stri.get(size_t(N) * Nr * cA);
+  }
+  for (int A0 = 0; A0 < Nd; A0 += cA) {
+    const int na = std::min(cA, Nd - A0);
+    const double* x3 = X3 + P * A0;
+    const double* xri = Nr > 0 ? X3ri + size_t(N) * Nr * A0 : nullptr;
+    if (!device_ptrs) {
+      UW12G_HIP(hipMemcpyAsync(st3.p, x3, sizeof(double) * P * na, hipMemcpyHostToDevice, s));
+      if (Nr > 0) UW12G_HIP(hipMemcpyAsync(stri.p, xri, sizeof(double) * size_t(N) * Nr * na, hipMemcpyHostToDevice, s));
+      x3 = st3.p;
+      xri = stri.p;
+    }
+    assemble_gamma(x3, xri, G0.p, N, Nr, na, s);
+    GemmDesc g;
+    g.M = int(MN);
+    g.N = D;
+    g.K = na;
+    g.alpha = 1.0;
+    g.beta = A0 == 0 ? 0.0 : 1.0;
+    g.A = G0.p;
+    g.Am = lin(1);
+    g.Ak = lin(int64_t(MN));
+    g.B = T.p + A0;
+    g.Bk = lin(1);
+    g.Bn = lin(Nd);
+    g.C = I->gamma.p;
+    g.Cm = lin(1);
+    g.Cn = lin(int64_t(MN));
+    gemm_f64(g, s, &ctx->gemm_ws);
+  }
+  UW12G_HIP(hipStreamSynchronize(s));
+  *out = I.release();
+}
+
+// ------------------------------------------------------------------ eval
+struct SpinData {
+  int o = 0, a = 0, c = 0;
+  double* Co = nullptr;  // N x o (device)
+  double* wk = nullptr;  // o   (1 + [k active]) when a > 0 else 1
+  double* phiW = nullptr;
+  double* phiV = nullptr;  // M o D
+  double* o2W = nullptr;
+  double* o2V = nullptr;  // a o D
+  double* phitV = nullptr;  // M a D_V
+  double* t = nullptr;      // D_W x D_V
+  double* x = nullptr;      // D_W x D_V
+};
+
+double spin_factor(int ns, int s, int t, double k0, double k1) {  // four_electron_utils.cpp:9-30
+  if (ns == 1) return 2.0 * (k0 + k1);
+  return s == t ? k1 : k0;
+}
+
+void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+              const uw12g_base_integrals* WV, const uw12g_abs_projectors* P, const double* C, size_t n_ao,
+              size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+              bool indirect_term, double k0, double k1, double* fock, double* terms, double* energy) {
+  require(ctx && W && V && WV && P, "null handle");
+  require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
+  require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");
+  const int ns = int(n_spin), N = int(n_ao);
+  require(W->N == N && V->N == N && WV->N == N && P->N == N, "Inconsistent number of atomic orbitals");
+  require(W->Nr > 0 && V->Nr > 0, "ri integrals requested, but no function provided to calculate");
+  require(W->Nr == P->Nr && V->Nr == P->Nr, "Total number of basis functions is inconsistent");
+  require(W->ctx == ctx && V->ctx == ctx && WV->ctx == ctx && P->ctx == ctx, "objects belong to another context");
+  set_device(ctx);
+  hipStream_t st = ctx->stream;
+  const int M = P->M, DW = W->D, DV = V->D, DWV = WV->D;
+  const bool calc_fock = fock != nullptr;
+  const bool do_ind = indirect_term && k1 != 0.0;
+  const double e = ns == 1 ? 2.0 : 1.0;
+  for (auto& r : ctx->prof) {
+    (void)hipEventDestroy(r.start);
+    (void)hipEventDestroy(r.stop);
+  }
+  ctx->prof.clear();
+
+  // ---- host: occupation-weighted orbitals and active block (utils.hpp:235-310)
+  std::vector<SpinData> sd(ns);
+  std::vector<std::vector<double>> Co_h(ns), wk_h(ns);
+  size_t occ_off = 0;
+  for (int s = 0; s < ns; ++s) {
+    const int o = int(n_occ[s]), a = int(n_active[s]);
+    require(size_t(o) <= n_orb, "more occupied orbitals than orbitals provided");
+    require(a <= o, "n_active exceeds the number of occupied orbitals");
+    const double g = ns == 1 ? 2.0 : 1.0;
+    sd[s].o = o;
+    sd[s].a = a;
+    sd[s].c = o - a;
+    Co_h[s].resize(size_t(N) * std::max(o, 1));
+    for (int k = 0; k < o; ++k) {
+      const double w = occ[occ_off + k];
+      require(w > 0, "occupations must be positive");
+      const double scl = std::sqrt(w / g);
+      for (int m = 0; m < N; ++m) Co_h[s][m + size_t(N) * k] = C[m + size_t(N) * (size_t(s) * n_orb + k)] * scl;
+    }
+    occ_off += o;
+    wk_h[s].assign(std::max(o, 1), 1.0);
+    if (a > 0)
+      for (int k = o - a; k < o; ++k) wk_h[s][k] = 2.0;
+  }
+  double* Eacc = ctx->buf("E", 8);
+  double* partial = ctx->buf("partial", kMaxPartials);
+  fill(Eacc, 8, 0.0, st);
+  double* Fraw = nullptr;
+  if (calc_fock) {
+    Fraw = ctx->buf("Fraw", size_t(N) * N * ns);
+    fill(Fraw, int64_t(N) * N * ns, 0.0, st);
+  }
+  for (int s = 0; s < ns; ++s) {
+    const int o = sd[s].o;
+    sd[s].Co = ctx->buf("Co" + std::to_string(s), size_t(N) * std::max(o, 1));
+    sd[s].wk = ctx->buf("wk" + std::to_string(s), std::max(o, 1));
+    UW12G_HIP(hipMemcpyAsync(sd[s].Co, Co_h[s].data(), sizeof(double) * N * std::max(o, 1), hipMemcpyHostToDevice, st));
+    UW12G_HIP(hipMemcpyAsync(sd[s].wk, wk_h[s].data(), sizeof(double) * std::max(o, 1), hipMemcpyHostToDevice, st));
+  }
+  const size_t NN = size_t(N) * N;
+
+  // ================= two-electron term (two_electron.cpp:11-152) =================
+  {
+    double* DCa = ctx->buf("DCa", NN * ns);
+    for (int s = 0; s < ns; ++s) {
+      const int a = sd[s].a;
+      const double* Ca = sd[s].Co + size_t(N) * sd[s].c;
+      GemmDesc g;  // DCa = Ca Ca^T
+      g.M = N;
+      g.N = N;
+      g.K = a;
+      g.A = Ca;
+      g.Am = lin(1);
+      g.Ak = lin(N);
+      g.B = Ca;
+      g.Bk = lin(N);
+      g.Bn = lin(1);
+      g.C = DCa + NN * s;
+      g.Cm = lin(1);
+      g.Cn = lin(N);
+      run_gemm(ctx, g, "2e_density");
+    }
+    // X_D[s]_A = sum_j (jj|A~)  (integrals.cpp:347-378); HBM-bound pass over Gamma_WV
+    double* XD = ctx->buf("XD", size_t(std::max(DWV, 1)) * ns);
+    timed(ctx, "2e_XD_gemv", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
+      gemv_t(WV->gamma.p, int64_t(NN), int64_t(NN), DCa, int64_t(NN), ns, XD, DWV, DWV, st);
+    });
+    std::vector<double> XDh(size_t(std::max(DWV, 1)) * ns);
+    UW12G_HIP(hipMemcpyAsync(XDh.data(), XD, sizeof(double) * DWV * ns, hipMemcpyDeviceToHost, st));
+    UW12G_HIP(hipStreamSynchronize(st));
+    double E2d = 0.0;
+    std::vector<double> vh(size_t(std::max(DWV, 1)) * ns, 0.0);
+    for (int s = 0; s < ns; ++s)
+      for (int t = 0; t < ns; ++t) {
+        const double f = spin_factor(ns, s, t, k0, k1);
+        if (f == 0) continue;
+        double d = 0.0;
+        for (int A = 0; A < DWV; ++A) d += XDh[size_t(DWV) * s + A] * WV->sign_host[A] * XDh[size_t(DWV) * t + A];
+        E2d += 0.5 * f * d;
+        for (int A = 0; A < DWV; ++A)
+          vh[size_t(DWV) * s + A] += 0.5 * ns * f * WV->sign_host[A] * XDh[size_t(DWV) * t + A];
+      }
+    terms[UW12G_E2_DIRECT] = E2d;
+    if (calc_fock && DWV > 0) {
+      double* vd = ctx->buf("XDv", size_t(DWV) * ns);
+      UW12G_HIP(hipMemcpyAsync(vd, vh.data(), sizeof(double) * DWV * ns, hipMemcpyHostToDevice, st));
+      timed(ctx, "2e_fock_gemv", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
+        gemv_n(WV->gamma.p, int64_t(NN), int64_t(NN), vd, DWV, ns, Fraw, int64_t(NN), DWV, st);
+      });
+    }
+    if (do_ind) {
+      for (int s = 0; s < ns; ++s) {
+        const int a = sd[s].a;
+        if (a == 0) continue;
+        const double* Ca = sd[s].Co + size_t(N) * sd[s].c;
+        double* P1 = ctx->buf("wv_phi", size_t(N) * a * DWV);
+        double* O2 = ctx->buf("wv_o2", size_t(a) * a * DWV);
+        GemmDesc g;  // P1[mu, j, A] = sum_nu Gamma_WV[mu, nu, A] Ca[nu, j]
+        g.M = N;
+        g.N = a;
+        g.K = N;
+        g.batch = DWV;
+        g.A = WV->gamma.p;
+        g.Am = lin(1);
+        g.Ak = lin(N);
+        g.Ab = int64_t(NN);
+        g.B = Ca;
+        g.Bk = lin(1);
+        g.Bn = lin(N);
+        g.C = P1;
+        g.Cm = lin(1);
+        g.Cn = lin(N);
+        g.Cb = int64_t(N) * a;
+        run_gemm(ctx, g, "2e_phi");
+        GemmDesc h;  // O2[i, (j,A)] = sum_mu Ca[mu,i] P1[mu,(j,A)]
+        h.M = a;
+        h.N = a * DWV;
+        h.K = N;
+        h.A = Ca;
+        h.Am = lin(N);
+        h.Ak = lin(1);
+        h.B = P1;
+        h.Bk = lin(1);
+        h.Bn = lin(N);
+        h.C = O2;
+        h.Cm = lin(1);
+        h.Cn = lin(a);
+        run_gemm(ctx, h, "2e_o2");
+        DotDesc dd;  // E -= 1/2 e k1 sum_A s_A |T_A|^2  (two_electron.cpp:112-114)
+        dd.x = O2;
+        dd.n = int64_t(a) * a * DWV;
+        dd.col_w = WV->sign.p;
+        dd.ld = int64_t(a) * a;
+        dd.col_n = DWV;
+        dot(dd, -0.5 * e * k1, Eacc + UW12G_E2_INDIRECT, 1, partial, st);
+        if (calc_fock) {  // F -= k1 R diag(s) R^T  (two_electron.cpp:116-128)
+          GemmDesc f;
+          f.M = N;
+          f.N = N;
+          f.K = a * DWV;
+          f.alpha = -k1;
+          f.beta = 1.0;
+          f.A = P1;
+          f.Am = lin(1);
+          f.Ak = lin(N);
+          f.ks = WV->sign.p;
+          f.Km = two(uint32_t(a), 0, 1);
+          f.B = P1;
+          f.Bk = lin(N);
+          f.Bn = lin(1);
+          f.C = Fraw + NN * s;
+          f.Cm = lin(1);
+          f.Cn = lin(N);
+          run_gemm(ctx, f, "2e_ind_fock");
+        }
+      }
+    }
+  }
+
+  // ================= W/V transforms: Phi, O2, t, PhiT_V, X =================
+  for (int s = 0; s < ns; ++s) {
+    SpinData& q = sd[s];
+    const int o = q.o, a = q.a, c = q.c;
+    const std::string sfx = std::to_string(s);
+    q.phiW = ctx->buf("phiW" + sfx, size_t(M) * o * DW);
+    q.phiV = ctx->buf("phiV" + sfx, size_t(M) * o * DV);
+    q.o2W = ctx->buf("o2W" + sfx, size_t(a) * o * DW);
+    q.o2V = ctx->buf("o2V" + sfx, size_t(a) * o * DV);
+    q.t = ctx->buf("t" + sfx, size_t(DW) * DV);
+    q.x = ctx->buf("x" + sfx, size_t(DW) * DV);
+    q.phitV = ctx->buf("phitV" + sfx, size_t(M) * a * DV);
+    const double* Ca = q.Co + size_t(N) * c;
+    for (int op = 0; op < 2; ++op) {
+      const uw12g_base_integrals* X = op == 0 ? W : V;
+      double* phi = op == 0 ? q.phiW : q.phiV;
+      double* o2 = op == 0 ? q.o2W : q.o2V;
+      GemmDesc g;  // Phi[mu', k, A] = sum_alpha Gamma[mu', alpha, A] Co[alpha, k]
+      g.M = M;
+      g.N = o;
+      g.K = N;
+      g.batch = X->D;
+      g.A = X->gamma.p;
+      g.Am = lin(1);
+      g.Ak = lin(M);
+      g.Ab = int64_t(M) * N;
+      g.B = q.Co;
+      g.Bk = lin(1);
+      g.Bn = lin(N);
+      g.C = phi;
+      g.Cm = lin(1);
+      g.Cn = lin(M);
+      g.Cb = int64_t(M) * o;
+      run_gemm(ctx, g, "phi_transform");
+      GemmDesc h;  // O2[i, (k, A)] = sum_mu Ca[mu, i] Phi[mu, (k, A)]
+      h.M = a;
+      h.N = o * X->D;
+      h.K = N;
+      h.A = Ca;
+      h.Am = lin(N);
+      h.Ak = lin(1);
+      h.B = phi;
+      h.Bk = lin(1);
+      h.Bn = lin(M);
+      h.C = o2;
+      h.Cm = lin(1);
+      h.Cn = lin(a);
+      run_gemm(ctx, h, "o2_transform");
+    }
+    GemmDesc t;  // t = O2_W^T O2_V  (four_electron_utils.cpp:33-49)
+    t.M = DW;
+    t.N = DV;
+    t.K = a * o;
+    t.A = q.o2W;
+    t.Am = lin(int64_t(a) * o);
+    t.Ak = lin(1);
+    t.B = q.o2V;
+    t.Bk = lin(1);
+    t.Bn = lin(int64_t(a) * o);
+    t.C = q.t;
+    t.Cm = lin(1);
+    t.Cn = lin(DW);
+    run_gemm(ctx, t, "4e_tab");
+    GemmDesc p;  // PhiT_V[mu', (i, B)] = sum_nu' S+[mu', nu'] Phi_V[nu', c+i, B]
+    p.M = M;
+    p.N = a * DV;
+    p.K = M;
+    p.A = P->Sp.p;
+    p.Am = lin(1);
+    p.Ak = lin(M);
+    p.B = q.phiV + size_t(M) * c;
+    p.Bk = lin(1);
+    p.Bn = two(uint32_t(std::max(a, 1)), M, int64_t(M) * o);
+    p.C = q.phitV;
+    p.Cm = lin(1);
+    p.Cn = lin(M);
+    run_gemm(ctx, p, "3e_abs_apply");
+    GemmDesc x;  // X[A, B] = sum_{mu', i} Phi_W[mu', c+i, A] PhiT_V[mu', i, B]  (direct_utils.cpp:62-133)
+    x.M = DW;
+    x.N = DV;
+    x.K = M * a;
+    x.A = q.phiW + size_t(M) * c;
+    x.Am = lin(int64_t(M) * o);
+    x.Ak = lin(1);
+    x.B = q.phitV;
+    x.Bk = lin(1);
+    x.Bn = lin(int64_t(M) * a);
+    x.C = q.x;
+    x.Cm = lin(1);
+    x.Cn = lin(DW);
+    run_gemm(ctx, x, "3e_xab");
+  }
+  // direct energies (four_electron.cpp:118-119, direct_utils.cpp:136-157)
+  for (int s = 0; s < ns; ++s)
+    for (int u = 0; u < ns; ++u) {
+      const double f = spin_factor(ns, s, u, k0, k1);
+      if (f == 0) continue;
+      DotDesc dd;
+      dd.n = int64_t(DW) * DV;
+      dd.row_w = W->sign.p;
+      dd.col_w = V->sign.p;
+      dd.ld = DW;
+      dd.col_n = DV;
+      dd.x = sd[u].t;
+      dd.y = sd[s].t;
+      dot(dd, 0.5 * f, Eacc + UW12G_E4_DIRECT, 1, partial, st);
+      dd.x = sd[s].x;
+      dd.y = sd[u].t;
+      dot(dd, -f, Eacc + UW12G_E3_DIRECT, 1, partial, st);
+    }
+
+  // ================= direct Fock: 4e + 3e (X dt~, t~ dX) =================
+  if (calc_fock) {
+    double* Tb = ctx->buf("Tb", size_t(DW) * DV);
+    double* Z = ctx->buf("Z", size_t(DW) * DV);
+    for (int s = 0; s < ns; ++s) {
+      SpinData& q = sd[s];
+      const int o = q.o;
+      const double fs = ns == 1 ? 0.5 : 1.0;
+      const double c0 = fs * spin_factor(ns, s, 0, k0, k1);
+      const double c1 = ns == 2 ? fs * spin_factor(ns, s, 1, k0, k1) : 0.0;
+      combine_dd(DW, DV, sd[0].t, ns == 2 ? sd[1].t : nullptr, sd[0].x, ns == 2 ? sd[1].x : nullptr, c0, c1,
+                 W->sign.p, V->sign.p, Tb, Z, st);
+      double* G = ctx->buf("G", size_t(N) * o * DV);
+      GemmDesc g;  // G[(beta,k), B] = sum_A W1[beta, k, A] Z[A, B]
+      g.M = N * o;
+      g.N = DV;
+      g.K = DW;
+      g.A = q.phiW;
+      g.Am = two(uint32_t(N), 1, M);
+      g.Ak = lin(int64_t(M) * o);
+      g.B = Z;
+      g.Bk = lin(1);
+      g.Bn = lin(DW);
+      g.C = G;
+      g.Cm = lin(1);
+      g.Cn = lin(int64_t(N) * o);
+      run_gemm(ctx, g, "direct_fock_G");
+      GemmDesc f;  // F += sum_{k,B} V1[alpha,k,B] w_k G[beta,k,B]  (four_electron.cpp:11-77, direct_utils.cpp:164-204)
+      f.M = N;
+      f.N = N;
+      f.K = o * DV;
+      f.beta = 1.0;
+      f.A = q.phiV;
+      f.Am = lin(1);
+      f.Ak = lin(M);
+      f.ks = q.wk;
+      f.Km = two(uint32_t(std::max(o, 1)), 1, 0);
+      f.B = G;
+      f.Bk = lin(N);
+      f.Bn = lin(1);
+      f.C = Fraw + NN * s;
+      f.Cm = lin(1);
+      f.Cn = lin(N);
+      run_gemm(ctx, f, "direct_fock_F");
+      // t~ dX/dD (direct_utils.cpp:242-347): F -= sum_AB Tb_AB Gamma_W[:, a, A]^T S+ Gamma_V[:, b, B]
+      const size_t MN = size_t(M) * N;
+      const int nb = int(std::max<size_t>(1, std::min<size_t>(DW, (size_t(1) << 27) / std::max<size_t>(MN, 1))));
+      double* Xi = ctx->buf("ttdx_xi", MN * nb);
+      double* Ps = ctx->buf("ttdx_psi", MN * nb);
+      for (int A0 = 0; A0 < DW; A0 += nb) {
+        const int na = std::min(nb, DW - A0);
+        GemmDesc a1;  // Xi[(nu', beta), A] = sum_B Gamma_V[(nu', beta), B] Tb[A0 + A, B]
+        a1.M = int(MN);
+        a1.N = na;
+        a1.K = DV;
+        a1.A = V->gamma.p;
+        a1.Am = lin(1);
+        a1.Ak = lin(int64_t(MN));
+        a1.B = Tb + A0;
+        a1.Bk = lin(DW);
+        a1.Bn = lin(1);
+        a1.C = Xi;
+        a1.Cm = lin(1);
+        a1.Cn = lin(int64_t(MN));
+        run_gemm(ctx, a1, "3e_ttdx_xi");
+        GemmDesc a2;  // Psi = S+ Xi
+        a2.M = M;
+        a2.N = N * na;
+        a2.K = M;
+        a2.A = P->Sp.p;
+        a2.Am = lin(1);
+        a2.Ak = lin(M);
+        a2.B = Xi;
+        a2.Bk = lin(1);
+        a2.Bn = lin(M);
+        a2.C = Ps;
+        a2.Cm = lin(1);
+        a2.Cn = lin(M);
+        run_gemm(ctx, a2, "3e_ttdx_abs");
+        GemmDesc a3;  // F -= sum_{mu', A} Gamma_W[mu', alpha, A0+A] Psi[mu', beta, A]
+        a3.M = N;
+        a3.N = N;
+        a3.K = M * na;
+        a3.alpha = -1.0;
+        a3.beta = 1.0;
+        a3.A = W->gamma.p + MN * A0;
+        a3.Am = lin(M);
+        a3.Ak = two(uint32_t(M), 1, int64_t(MN));
+        a3.B = Ps;
+        a3.Bk = two(uint32_t(M), 1, int64_t(MN));
+        a3.Bn = lin(M);
+        a3.C = Fraw + NN * s;
+        a3.Cm = lin(1);
+        a3.Cn = lin(N);
+        run_gemm(ctx, a3, "3e_ttdx_fock");
+      }
+    }
+  }
+
+  // ================= same-spin (indirect) terms, x k1 =================
+  if (do_ind) {
+    const size_t MN = size_t(M) * N;
+    for (int s = 0; s < ns; ++s) {
+      SpinData& q = sd[s];
+      const int o = q.o, a = q.a, c = q.c;
+      if (a == 0 || o == 0) continue;
+      const int64_t ao = int64_t(a) * o;
+      // signed two-index transforms  W2s = sW o O2_W, V2s = sV o O2_V
+      double* W2s = ctx->buf("W2s", size_t(ao) * DW);
+      double* V2s = ctx->buf("V2s", size_t(ao) * DV);
+      scale_cols(q.o2W, ao, DW, W->sign.p, W2s, st);
+      scale_cols(q.o2V, ao, DV, V->sign.p, V2s, st);
+      // ---- 4e indirect energy (four_electron.cpp:143-179, 274-278)
+      {
+        double* W4 = ctx->buf("W4", size_t(ao) * ao);
+        double* V4 = ctx->buf("V4", size_t(ao) * ao);
+        for (int op = 0; op < 2; ++op) {
+          GemmDesc g;  // X4 = O2 diag(s) O2^T
+          g.M = int(ao);
+          g.N = int(ao);
+          g.K = op == 0 ? DW : DV;
+          g.A = op == 0 ? W2s : V2s;
+          g.Am = lin(1);
+          g.Ak = lin(ao);
+          g.B = op == 0 ? q.o2W : q.o2V;
+          g.Bk = lin(ao);
+          g.Bn = lin(1);
+          g.C = op == 0 ? W4 : V4;
+          g.Cm = lin(1);
+          g.Cn = lin(ao);
+          run_gemm(ctx, g, "4e_ind_x4");
+        }
+        timed(ctx, "4e_ind_energy", 2.0 * ao * ao, 16.0 * ao * ao,
+              [&] { e4i_energy(W4, V4, a, o, -0.5 * e * k1, Eacc + UW12G_E4_INDIRECT, partial, st); });
+        ctx->release("W4");
+        ctx->release("V4");
+      }
+      // ---- 4e indirect Fock (four_electron.cpp:181-249)
+      if (calc_fock) {
+        const size_t nXY = size_t(N) * ao * a;
+        double* Xb = ctx->buf("f4_x", nXY);
+        double* Yb = ctx->buf("f4_y", nXY);
+        GemmDesc gx;  // X_i[alpha, (j,k)] = sum_A Phi_W[alpha, c+i, A] sW_A O2_W[(j,k), A]
+        gx.M = N;
+        gx.N = int(ao);
+        gx.K = DW;
+        gx.batch = a;
+        gx.A = q.phiW + size_t(M) * c;
+        gx.Am = lin(1);
+        gx.Ak = lin(int64_t(M) * o);
+        gx.Ab = M;
+        gx.B = W2s;
+        gx.Bk = lin(ao);
+        gx.Bn = lin(1);
+        gx.C = Xb;
+        gx.Cm = lin(1);
+        gx.Cn = lin(N);
+        gx.Cb = int64_t(N) * ao;
+        run_gemm(ctx, gx, "4e_ind_fock_x");
+        GemmDesc gy;  // Y_i[(beta, j), k] = sum_B Phi_V[beta, c+j, B] sV_B O2_V[i + a k, B]
+        gy.M = N * a;
+        gy.N = o;
+        gy.K = DV;
+        gy.batch = a;
+        gy.A = q.phiV + size_t(M) * c;
+        gy.Am = two(uint32_t(N), 1, M);
+        gy.Ak = lin(int64_t(M) * o);
+        gy.B = V2s;
+        gy.Bk = lin(ao);
+        gy.Bn = lin(a);
+        gy.Bb = 1;
+        gy.C = Yb;
+        gy.Cm = lin(1);
+        gy.Cn = lin(int64_t(N) * a);
+        gy.Cb = int64_t(N) * ao;
+        run_gemm(ctx, gy, "4e_ind_fock_y");
+        const double fz = c > 0 ? 0.5 : 1.0;  // frozen core: 1/2 (F_act + F_frozen)
+        GemmDesc gf;  // F -= 2 k1 sum_{(j,k,i)} X[alpha, .] Y[beta, .]
+        gf.M = N;
+        gf.N = N;
+        gf.K = int(ao * a);
+        gf.alpha = -2.0 * k1 * fz;
+        gf.beta = 1.0;
+        gf.A = Xb;
+        gf.Am = lin(1);
+        gf.Ak = lin(N);
+        gf.B = Yb;
+        gf.Bk = lin(N);
+        gf.Bn = lin(1);
+        gf.C = Fraw + NN * s;
+        gf.Cm = lin(1);
+        gf.Cn = lin(N);
+        run_gemm(ctx, gf, "4e_ind_fock_f");
+        if (c > 0) {
+          const size_t nXY2 = size_t(N) * ao * o;
+          double* X2b = ctx->buf("f4_x2", nXY2);
+          double* Y2b = ctx->buf("f4_y2", nXY2);
+          GemmDesc hx = gx;  // X'_l for every occupied l
+          hx.batch = o;
+          hx.A = q.phiW;
+          hx.C = X2b;
+          run_gemm(ctx, hx, "4e_ind_fock_x");
+          GemmDesc hy;  // Y'_l[(beta, k), j] = sum_B Phi_V[beta, k, B] sV_B O2_V[j + a l, B]
+          hy.M = N * o;
+          hy.N = a;
+          hy.K = DV;
+          hy.batch = o;
+          hy.A = q.phiV;
+          hy.Am = two(uint32_t(N), 1, M);
+          hy.Ak = lin(int64_t(M) * o);
+          hy.B = V2s;
+          hy.Bk = lin(ao);
+          hy.Bn = lin(1);
+          hy.Bb = a;
+          hy.C = Y2b;
+          hy.Cm = two(uint32_t(N), 1, int64_t(N) * a);
+          hy.Cn = lin(N);
+          hy.Cb = int64_t(N) * ao;
+          run_gemm(ctx, hy, "4e_ind_fock_y");
+          GemmDesc hf = gf;
+          hf.K = int(ao * o);
+          hf.A = X2b;
+          hf.B = Y2b;
+          run_gemm(ctx, hf, "4e_ind_fock_f");
+          ctx->release("f4_x2");
+          ctx->release("f4_y2");
+        }
+        ctx->release("f4_x");
+        ctx->release("f4_y");
+      }
+      // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k
+      {
+        const int64_t Ma = int64_t(M) * a;
+        const size_t cap = size_t(1) << 27;  // doubles per block buffer (1 GiB)
+        int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
+        double* GWb = ctx->buf("e3_gw", size_t(Ma) * a * nk);
+        double* GVb = ctx->buf("e3_gv", size_t(Ma) * a * nk);
+        for (int k0 = 0; k0 < o; k0 += nk) {
+          const int kn = std::min(nk, o - k0);
+          GemmDesc g;  // GW[(mu', i), (j, k)] = sum_A Phi_W[mu', c+i, A] W2s[(j, k), A]
+          g.M = int(Ma);
+          g.N = a * kn;
+          g.K = DW;
+          g.A = q.phiW + size_t(M) * c;
+          g.Am = lin(1);
+          g.Ak = lin(int64_t(M) * o);
+          g.B = W2s + int64_t(a) * k0;
+          g.Bk = lin(ao);
+          g.Bn = lin(1);
+          g.C = GWb;
+          g.Cm = lin(1);
+          g.Cn = lin(Ma);
+          run_gemm(ctx, g, "3e_ind_e_gw");
+          GemmDesc h = g;  // GV[(mu', j), (i, k)] = sum_B PhiT_V[mu', j, B] V2s[(i, k), B]
+          h.K = DV;
+          h.A = q.phitV;
+          h.Ak = lin(Ma);
+          h.B = V2s + int64_t(a) * k0;
+          h.C = GVb;
+          run_gemm(ctx, h, "3e_ind_e_gv");
+          timed(ctx, "3e_ind_energy", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
+                [&] { e3i_energy(GWb, GVb, M, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
+        }
+        ctx->release("e3_gw");
+        ctx->release("e3_gv");
+      }
+      // ---- 3e indirect Fock (indirect_utils.cpp:138-345)
+      if (calc_fock) {
+        double* phitW = ctx->buf("phitW", size_t(M) * a * DW);
+        GemmDesc p;  // PhiT_W = S+ Phi_W[:, act]
+        p.M = M;
+        p.N = a * DW;
+        p.K = M;
+        p.A = P->Sp.p;
+        p.Am = lin(1);
+        p.Ak = lin(M);
+        p.B = q.phiW + size_t(M) * c;
+        p.Bk = lin(1);
+        p.Bn = two(uint32_t(a), M, int64_t(M) * o);
+        p.C = phitW;
+        p.Cm = lin(1);
+        p.Cn = lin(M);
+        run_gemm(ctx, p, "3e_abs_apply");
+        double* L = ctx->buf("L", MN * std::max(DW, DV));
+        const int64_t MNo = int64_t(MN) * o;
+        const size_t cap = size_t(1) << 28;  // doubles (2 GiB) for the K~ block
+        const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
+        double* Kt = ctx->buf("Kt", size_t(MNo) * nj);
+        // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
+        for (int pass = 0; pass < 2; ++pass) {
+          const uw12g_base_integrals* X = pass == 0 ? W : V;
+          const uw12g_base_integrals* Y = pass == 0 ? V : W;
+          const double* phitY = pass == 0 ? q.phitV : phitW;
+          const double* phiY = pass == 0 ? q.phiV : q.phiW;
+          const double* X2s = pass == 0 ? W2s : V2s;
+          const int DX = X->D, DY = Y->D;
+          for (int j0 = 0; j0 < a; j0 += nj) {
+            const int jn = std::min(nj, a - j0);
+            GemmDesc g;  // K~[mu', (beta, k) | j] = sum_B PhiT_Y[mu', j, B] sY_B Phi_Y[beta, k, B]
+            g.M = M;
+            g.N = N * o;
+            g.K = DY;
+            g.batch = jn;
+            g.A = phitY + size_t(M) * j0;
+            g.Am = lin(1);
+            g.Ak = lin(int64_t(M) * a);
+            g.Ab = M;
+            g.ks = Y->sign.p;
+            g.Km = lin(1);
+            g.B = phiY;
+            g.Bk = lin(int64_t(M) * o);
+            g.Bn = two(uint32_t(N), 1, M);
+            g.C = Kt;
+            g.Cm = lin(1);
+            g.Cn = lin(M);
+            g.Cb = MNo;
+            run_gemm(ctx, g, "3e_ind_fock_kt");
+            GemmDesc h;  // L[(mu', beta), A] += sum_{(k, j)} K~[(mu', beta), k | j] X2s[j + a k, A]
+            h.M = int(MN);
+            h.N = DX;
+            h.K = o * jn;
+            h.beta = j0 == 0 ? 0.0 : 1.0;
+            h.A = Kt;
+            h.Am = lin(1);
+            h.Ak = lin(int64_t(MN));
+            h.B = X2s + j0;
+            h.Bk = two(uint32_t(o), a, 1);
+            h.Bn = lin(ao);
+            h.C = L;
+            h.Cm = lin(1);
+            h.Cn = lin(int64_t(MN));
+            run_gemm(ctx, h, "3e_ind_fock_l");
+          }
+          GemmDesc f;  // F += k1 sum_{mu', A} Gamma_X[mu', alpha, A] L[mu', beta, A]
+          f.M = N;
+          f.N = N;
+          f.K = M * DX;
+          f.alpha = k1;
+          f.beta = 1.0;
+          f.A = X->gamma.p;
+          f.Am = lin(M);
+          f.Ak = two(uint32_t(M), 1, int64_t(MN));
+          f.B = L;
+          f.Bk = two(uint32_t(M), 1, int64_t(MN));
+          f.Bn = lin(M);
+          f.C = Fraw + NN * s;
+          f.Cm = lin(1);
+          f.Cn = lin(N);
+          run_gemm(ctx, f, "3e_ind_fock_f");
+        }
+        ctx->release("L");
+        ctx->release("Kt");
+        // fock_k (indirect_utils.cpp:199-259), blocked over j
+        const int64_t MaN = int64_t(M) * a * N;
+        const int nj2 = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MaN, 1))));
+        double* Qt = ctx->buf("Qt", size_t(MaN) * nj2);
+        double* R = ctx->buf("R", size_t(DW) * N * nj2);
+        for (int j0 = 0; j0 < a; j0 += nj2) {
+          const int jn = std::min(nj2, a - j0);
+          GemmDesc g;  // Q~[mu', i, alpha | j] = sum_B PhiT_V[mu', j, B] sV_B Phi_V[alpha, c+i, B]
+          g.M = M;
+          g.N = N * a;
+          g.K = DV;
+          g.batch = jn;
+          g.A = q.phitV + size_t(M) * j0;
+          g.Am = lin(1);
+          g.Ak = lin(int64_t(M) * a);
+          g.Ab = M;
+          g.ks = V->sign.p;
+          g.Km = lin(1);
+          g.B = q.phiV + size_t(M) * c;
+          g.Bk = lin(int64_t(M) * o);
+          g.Bn = two(uint32_t(N), 1, M);
+          g.C = Qt;
+          g.Cm = lin(1);
+          g.Cn = two(uint32_t(N), int64_t(M) * a, M);
+          g.Cb = MaN;
+          run_gemm(ctx, g, "3e_ind_fock_q");
+          GemmDesc h;  // R[A, alpha | j] = sum_{(mu', i)} Phi_W[mu', c+i, A] Q~[(mu', i), alpha | j]
+          h.M = DW;
+          h.N = N;
+          h.K = M * a;
+          h.batch = jn;
+          h.A = q.phiW + size_t(M) * c;
+          h.Am = lin(int64_t(M) * o);
+          h.Ak = lin(1);
+          h.B = Qt;
+          h.Bk = lin(1);
+          h.Bn = lin(int64_t(M) * a);
+          h.Bb = MaN;
+          h.C = R;
+          h.Cm = lin(1);
+          h.Cn = lin(DW);
+          h.Cb = int64_t(DW) * N;
+          run_gemm(ctx, h, "3e_ind_fock_r");
+          GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
+          f.M = N;
+          f.N = N;
+          f.K = DW * jn;
+          f.alpha = k1;
+          f.beta = 1.0;
+          f.A = q.phiW + size_t(M) * (c + j0);
+          f.Am = lin(1);
+          f.Ak = two(uint32_t(DW), int64_t(M) * o, M);
+          f.ks = W->sign.p;
+          f.Km = two(uint32_t(DW), 1, 0);
+          f.B = R;
+          f.Bk = two(uint32_t(DW), 1, int64_t(DW) * N);
+          f.Bn = lin(DW);
+          f.C = Fraw + NN * s;
+          f.Cm = lin(1);
+          f.Cn = lin(N);
+          run_gemm(ctx, f, "3e_ind_fock_k");
+        }
+        ctx->release("Qt");
+        ctx->release("R");
+        ctx->release("phitW");
+      }
+    }
+  }
+
+  // ---- results
+  double Eh[8];
+  UW12G_HIP(hipMemcpyAsync(Eh, Eacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
+  if (calc_fock) {
+    double* Fs = ctx->buf("Fsym", NN * ns);
+    for (int s = 0; s < ns; ++s) symmetrise_add(Fraw + NN * s, Fs + NN * s, N, 0, st);
+    UW12G_HIP(hipMemcpyAsync(fock, Fs, sizeof(double) * NN * ns, hipMemcpyDeviceToHost, st));
+  }
+  UW12G_HIP(hipStreamSynchronize(st));
+  for (int i = 1; i < 6; ++i) terms[i] = Eh[i];
+  if (energy) {
+    double tot = 0.0;
+    for (int i = 0; i < 6; ++i) tot += terms[i];
+    *energy = tot;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" {
+
+const char* uw12g_last_error(void) { return g_last_error.c_str(); }
+
+int uw12g_context_create(int device, uw12g_context** out) {
+  return guard([&] {
+    require(out != nullptr, "null output handle");
+    int count = 0;
+    UW12G_HIP(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, "invalid device index");
+    std::unique_ptr<uw12g_context> c(new uw12g_context());
+    c->device = device;
+    UW12G_HIP(hipSetDevice(device));
+    UW12G_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Error(-3, "rocblas_create_handle failed");
+    rocblas_set_stream(c->rb, c->stream);
+    *out = c.release();
+  });
+}
+
+void uw12g_context_destroy(uw12g_context* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  delete ctx;
+}
+
+int uw12g_setup_base_integrals(uw12g_context* ctx, const double* X3, const double* X2, const double* X3_ri,
+                               size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out) {
+  return guard([&] { setup_base(ctx, X3, X2, X3_ri, n_ao, n_df, n_ri, false, out); });
+}
+
+int uw12g_setup_base_integrals_device(uw12g_context* ctx, const double* X3_dev, const double* X2,
+                                      const double* X3_ri_dev, size_t n_ao, size_t n_df, size_t n_ri,
+                                      uw12g_base_integrals** out) {
+  return guard([&] { setup_base(ctx, X3_dev, X2, X3_ri_dev, n_ao, n_df, n_ri, true, out); });
+}
+
+void uw12g_base_integrals_destroy(uw12g_base_integrals* ints) {
+  if (!ints) return;
+  (void)hipSetDevice(ints->ctx->device);
+  (void)hipStreamSynchronize(ints->ctx->stream);
+  delete ints;
+}
+
+long uw12g_base_integrals_kept(const uw12g_base_integrals* ints) { return ints ? long(ints->D) : -1; }
+
+int uw12g_setup_abs_projectors(uw12g_context* ctx, const double* S, size_t n_ao, size_t n_ri,
+                               uw12g_abs_projectors** out) {
+  return guard([&] {
+    require(ctx && S && out, "null argument");
+    require(n_ao > 0, "Number of ao orbitals is zero");
+    const size_t Mz = n_ao + n_ri;
+    require(Mz < 65536, "size out of range", -2);
+    require(is_symmetric(S, Mz, 1e-10), "matrix must be symmetric");
+    set_device(ctx);
+    const int M = int(Mz);
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<uw12g_abs_projectors> P(new uw12g_abs_projectors());
+    P->ctx = ctx;
+    P->N = int(n_ao);
+    P->Nr = int(n_ri);
+    P->M = M;
+    DBuf Vd;
+    Vd.get(Mz * Mz);
+    UW12G_HIP(hipMemcpyAsync(Vd.p, S, sizeof(double) * Mz * Mz, hipMemcpyHostToDevice, s));
+    std::vector<double> w = eig_device(ctx, Vd.p, M);
+    double mx = 0.0;
+    for (double x : w) mx = std::max(mx, std::fabs(x));
+    const double thr = std::max(mx * 1e-8, 1e-6);  // ri_utils.hpp:49-50, linalg.hpp:1166-1169
+    std::vector<int> keep;
+    std::vector<double> ones, inv;
+    for (int i = 0; i < M; ++i)
+      if (std::fabs(w[i]) > thr) {
+        keep.push_back(i);
+        ones.push_back(1.0);
+        inv.push_back(1.0 / w[i]);
+      }
+    const int K = int(keep.size());
+    P->Sp.get(Mz * Mz);
+    if (K == 0) {
+      fill(P->Sp.p, int64_t(Mz * Mz), 0.0, s);
+    } else {
+      int* keep_d = nullptr;
+      UW12G_HIP(hipMalloc(&keep_d, sizeof(int) * K));
+      DBuf on, iv, Vk, Vs;
+      on.get(K);
+      iv.get(K);
+      Vk.get(Mz * K);
+      Vs.get(Mz * K);
+      UW12G_HIP(hipMemcpyAsync(keep_d, keep.data(), sizeof(int) * K, hipMemcpyHostToDevice, s));
+      UW12G_HIP(hipMemcpyAsync(on.p, ones.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
+      UW12G_HIP(hipMemcpyAsync(iv.p, inv.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
+      gather_scale_cols(Vd.p, M, keep_d, on.p, Vk.p, K, s);
+      gather_scale_cols(Vd.p, M, keep_d, iv.p, Vs.p, K, s);
+      GemmDesc g;  // S+ = V_k diag(1/s) V_k^T
+      g.M = M;
+      g.N = M;
+      g.K = K;
+      g.A = Vs.p;
+      g.Am = lin(1);
+      g.Ak = lin(M);
+      g.B = Vk.p;
+      g.Bk = lin(M);
+      g.Bn = lin(1);
+      g.C = P->Sp.p;
+      g.Cm = lin(1);
+      g.Cn = lin(M);
+      gemm_f64(g, s, &ctx->gemm_ws);
+      UW12G_HIP(hipStreamSynchronize(s));
+      (void)hipFree(keep_d);
+    }
+    UW12G_HIP(hipStreamSynchronize(s));
+    *out = P.release();
+  });
+}
+
+void uw12g_abs_projectors_destroy(uw12g_abs_projectors* abs) {
+  if (!abs) return;
+  (void)hipSetDevice(abs->ctx->device);
+  (void)hipStreamSynchronize(abs->ctx->stream);
+  delete abs;
+}
+
+int uw12g_abs_projectors_get(const uw12g_abs_projectors* abs, double* out) {
+  return guard([&] {
+    require(abs && out, "null argument");
+    set_device(abs->ctx);
+    UW12G_HIP(hipMemcpyAsync(out, abs->Sp.p, sizeof(double) * size_t(abs->M) * abs->M, hipMemcpyDeviceToHost,
+                             abs->ctx->stream));
+    UW12G_HIP(hipStreamSynchronize(abs->ctx->stream));
+  });
+}
+
+int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                    const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                    size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                    int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
+                    double* fock, double* terms_out, double* energy) {
+  return guard([&] {
+    require(precision == UW12G_FP64, "unsupported precision", -2);
+    double t6[6] = {0, 0, 0, 0, 0, 0};
+    double* tp = terms_out ? terms_out : t6;
+    evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
+             scale_opp_spin, scale_same_spin, fock, tp, energy);
+  });
+}
+
+static bool nearly_zero(double v) { return std::fabs(v) <= 4 * DBL_EPSILON; }  // utils.hpp:186-189
+
+int uw12g_energy(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                 const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                 size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                 double scale_opp_spin, double scale_same_spin, double* energy) {
+  return uw12g_form_fock(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active,
+                         !nearly_zero(scale_same_spin), scale_opp_spin, scale_same_spin, UW12G_FP64, nullptr,
+                         nullptr, energy);
+}
+
+int uw12g_fock(uw12g_context* ctx, double* fock, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+               const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+               size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+               double scale_opp_spin, double scale_same_spin, double* energy) {
+  if (!fock) {
+    g_last_error = "null fock buffer";
+    return -1;
+  }
+  return uw12g_form_fock(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active,
+                         !nearly_zero(scale_same_spin), scale_opp_spin, scale_same_spin, UW12G_FP64, fock, nullptr,
+                         energy);
+}
+
+int uw12g_context_set_profiling(uw12g_context* ctx, int on) {
+  return guard([&] {
+    require(ctx != nullptr, "null context");
+    ctx->profiling = on != 0;
+  });
+}
+
+int uw12g_context_profile(const uw12g_context* ctx, const char** names, double* ms, double* flops, double* bytes,
+                          int cap) {
+  if (!ctx) return -1;
+  const int n = int(ctx->prof.size());
+  for (int i = 0; i < n && i < cap; ++i) {
+    const ProfileRec& r = ctx->prof[i];
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, r.start, r.stop) != hipSuccess) t = -1.f;
+    if (names) names[i] = r.name.c_str();
+    if (ms) ms[i] = t;
+    if (flops) flops[i] = r.flops;
+    if (bytes) bytes[i] = r.bytes;
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,121 @@
+// Internal declarations shared by the HIP translation units of libuw12g.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace uw12g {
+
+// Exact unsigned 32-bit division by a run-time constant d >= 1
+// (round-up multiplier method, Granlund & Montgomery, PLDI 1994):
+//   l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1,
+//   x / d = (umulhi(x, m) + x) >> l   for all x < 2^32.
+struct FastDiv {
+  uint32_t d = 1, m = 0, s = 0;
+};
+
+inline FastDiv make_fastdiv(uint32_t d) {
+  FastDiv f;
+  f.d = d;
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  f.s = l;
+  f.m = uint32_t(((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1);
+  return f;
+}
+
+__host__ __device__ inline uint32_t fd_div(uint32_t x, const FastDiv& f) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t hi = __umulhi(x, f.m);
+#else
+  const uint32_t hi = uint32_t((uint64_t(x) * f.m) >> 32);
+#endif
+  return uint32_t((uint64_t(hi) + x) >> f.s);
+}
+
+// Two-level index map: off(i) = (i mod n0) s0 + (i div n0) s1.
+// n0 == 0 marks a plain one-level map off(i) = i s0.
+struct Map2 {
+  int64_t s0 = 0, s1 = 0;
+  uint32_t n0 = 0;
+  FastDiv fd;
+  __host__ __device__ inline int64_t off(uint32_t i) const {
+    if (n0 == 0) return int64_t(i) * s0;
+    const uint32_t q = fd_div(i, fd);
+    return int64_t(i - q * n0) * s0 + int64_t(q) * s1;
+  }
+};
+
+inline Map2 lin(int64_t s) {
+  Map2 m;
+  m.s0 = s;
+  return m;
+}
+
+inline Map2 two(uint32_t n0, int64_t s0, int64_t s1) {
+  Map2 m;
+  m.s0 = s0;
+  m.s1 = s1;
+  m.n0 = n0;
+  m.fd = make_fastdiv(n0);
+  return m;
+}
+
+// C(m,n) <- alpha sum_k A(m,k) ks(k) B(k,n) + beta C(m,n), for every batch b.
+// A(m,k) = A[Ab*b + Am.off(m) + Ak.off(k)], and likewise for B, C, ks.
+struct GemmDesc {
+  int M = 0, N = 0, K = 0, batch = 1;
+  double alpha = 1.0, beta = 0.0;
+  const double* A = nullptr;
+  Map2 Am, Ak;
+  int64_t Ab = 0;
+  const double* B = nullptr;
+  Map2 Bk, Bn;
+  int64_t Bb = 0;
+  double* C = nullptr;
+  Map2 Cm, Cn;
+  int64_t Cb = 0;
+  const double* ks = nullptr;
+  Map2 Km;
+  int64_t Kb = 0;
+};
+
+// Allocation registry: every device buffer the library allocates is recorded
+// so that each launch can check the extent of its operands on the host
+// before it reaches the GPU.
+void register_alloc(const void* p, size_t bytes);
+void unregister_alloc(const void* p);
+// Throws Error(-2) unless [base, base + (max_off + 1) doubles) lies inside one
+// registered allocation; unregistered bases (caller-owned memory) pass.
+void check_extent(const void* base, int64_t max_off, const char* what);
+// Largest offset a map produces over indices [0, extent).
+int64_t map_max(const Map2& m, int64_t extent);
+
+struct Workspace;
+void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws);
+void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
+
+// Device scratch owned by a context: grown on demand, reused across calls.
+struct Workspace {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  double* get(size_t n_doubles, hipStream_t stream);  // drains `stream` before growing
+  ~Workspace();
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define UW12G_HIP(call)                                                                    \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      throw ::uw12g::Error(-3, std::string(#call " failed: ") + hipGetErrorString(e_));   \
+  } while (0)
+
+}  // namespace uw12g

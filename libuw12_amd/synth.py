@@ -1,0 +1,198 @@
+"""Synthetic UW12 inputs: s-Gaussian clusters (the benchmark configurations)
+and small well-conditioned random systems (the style of the reference's own
+unit-test fixtures, tests/setup_integrals.hpp, src/utils/linalg.hpp:277-292,507-516).
+
+The integrals are computed by libuw12_synth.so (csrc/synth.cpp, host OpenMP);
+geometry, bases and orbitals are built here with numpy's PCG64 generator.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, 'libuw12_synth.so')
+_lib = None
+
+# per-atom basis recipe: (count, first exponent, ratio)
+AO = (8, 0.1, 2.5)
+DF = (24, 0.2, 2.0)
+RI = (32, 0.15, 1.8)
+GEMINAL = ((0.4, 0.6), (0.5, 2.0))  # W = -sum c_i exp(-g_i r^2)
+
+# name -> (atoms, box, min separation, (n_alpha, n_beta or None), truncate last atom to (ao, df, ri))
+CONFIGS = {
+    'c1': dict(atoms=3, box=4.0, min_sep=2.0, occ=(5, None)),
+    'c2': dict(atoms=40, box=20.0, min_sep=2.5, occ=(80, None)),
+    'c3': dict(atoms=100, box=30.0, min_sep=2.5, occ=(160, 150)),
+    'c4': dict(atoms=100, box=30.0, min_sep=2.5, occ=(160, None)),
+    'c5': dict(atoms=188, box=40.0, min_sep=2.5, occ=(300, 290), last=(4, 12, 16)),
+}
+CONFIG_INDEX = {'c1': 1, 'c2': 2, 'c3': 3, 'c4': 4, 'c5': 5}
+
+
+def build():
+    subprocess.check_call(['g++', '-O2', '-std=c++17', '-fopenmp', '-fPIC', '-shared', '-o', LIB,
+                           os.path.join(HERE, 'csrc', 'synth.cpp')])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.synth_set_geminal.argtypes = [ctypes.c_int, dp, dp]
+        L.synth_overlap.argtypes = [dp, dp, ctypes.c_int, dp]
+        L.synth_two_index.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int, dp]
+        L.synth_three_index.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ctypes.c_int, dp]
+        L.synth_three_index_ri.argtypes = [ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp,
+                                           ctypes.c_int, dp]
+        L.synth_pair_integral.argtypes = [ctypes.c_int, ctypes.c_double, dp, ctypes.c_double, dp]
+        L.synth_pair_integral.restype = ctypes.c_double
+        c = np.array(GEMINAL[0], dtype=np.float64)
+        g = np.array(GEMINAL[1], dtype=np.float64)
+        L.synth_set_geminal(len(c), c.ctypes.data_as(dp), g.ctypes.data_as(dp))
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+KINDS = {'V': 0, 'W': 1, 'WV': 2}
+
+
+def geometry(n_atoms, box, min_sep, seed):
+    """Uniform positions in a cube with a minimum separation (rejection sampling)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pts = []
+    tries = 0
+    while len(pts) < n_atoms:
+        p = rng.uniform(0.0, box, 3)
+        tries += 1
+        if tries > 10_000_000:
+            raise RuntimeError('could not place atoms')
+        if all(np.sum((p - q) ** 2) >= min_sep ** 2 for q in pts):
+            pts.append(p)
+    return np.array(pts)
+
+
+def basis(atoms, recipe, last=None):
+    n, a0, r = recipe
+    ex = a0 * r ** np.arange(n)
+    exps, cens = [], []
+    for i, at in enumerate(atoms):
+        e = ex if (last is None or i < len(atoms) - 1) else ex[:last]  # most diffuse first
+        exps.append(e)
+        cens.append(np.repeat(at[None, :], len(e), axis=0))
+    return np.ascontiguousarray(np.concatenate(exps)), np.ascontiguousarray(np.concatenate(cens))
+
+
+def overlap(ex, cen):
+    n = len(ex)
+    S = np.zeros((n, n), order='F')
+    lib().synth_overlap(_p(ex), _p(cen), n, _p(S))
+    return 0.5 * (S + S.T)
+
+
+def two_index(kind, ex, cen):
+    n = len(ex)
+    X = np.zeros((n, n), order='F')
+    lib().synth_two_index(KINDS[kind], _p(ex), _p(cen), n, _p(X))
+    return np.asfortranarray(0.5 * (X + X.T))
+
+
+def three_index(kind, eao, cao, edf, cdf):
+    N, Nd = len(eao), len(edf)
+    X = np.zeros((N * (N + 1) // 2, Nd), order='F')
+    lib().synth_three_index(KINDS[kind], _p(eao), _p(cao), N, _p(edf), _p(cdf), Nd, _p(X))
+    return X
+
+
+def three_index_ri(kind, eri, cri, eao, cao, edf, cdf):
+    N, Nr, Nd = len(eao), len(eri), len(edf)
+    X = np.zeros((N * Nr, Nd), order='F')
+    lib().synth_three_index_ri(KINDS[kind], _p(eri), _p(cri), Nr, _p(eao), _p(cao), N, _p(edf), _p(cdf), Nd,
+                               _p(X))
+    return X
+
+
+def lowdin_orbitals(S_ao, n_occ, rng):
+    G = rng.standard_normal((S_ao.shape[0], n_occ))
+    w, U = np.linalg.eigh(G.T @ S_ao @ G)
+    return np.asfortranarray(G @ (U / np.sqrt(w)) @ U.T)
+
+
+def make_system(name, with_integrals=True, seed_offset=0):
+    """Inputs of a named benchmark configuration as a dict with keys
+    N, Nd, Nr, S, ints {'W','V','WV': (X3, X2, X3_ri)}, orbitals, occ, n_active, atoms."""
+    cfg = CONFIGS[name]
+    base = 1000 * CONFIG_INDEX[name] + seed_offset
+    atoms = geometry(cfg['atoms'], cfg['box'], cfg['min_sep'], base + 0)
+    last = cfg.get('last')
+    eao, cao = basis(atoms, AO, last[0] if last else None)
+    edf, cdf = basis(atoms, DF, last[1] if last else None)
+    eri, cri = basis(atoms, RI, last[2] if last else None)
+    N, Nd, Nr = len(eao), len(edf), len(eri)
+    sys_ = dict(name=name, N=N, Nd=Nd, Nr=Nr, atoms=atoms, basis=dict(ao=(eao, cao), df=(edf, cdf), ri=(eri, cri)))
+    S = overlap(np.concatenate([eao, eri]), np.concatenate([cao, cri]))
+    sys_['S'] = S
+    rng = np.random.Generator(np.random.PCG64(base + 1))
+    na, nb = cfg['occ']
+    Ca = lowdin_orbitals(S[:N, :N], na, rng)
+    if nb is None:
+        sys_['orbitals'] = [Ca]
+        sys_['occ'] = [np.full(na, 2.0)]
+        sys_['n_active'] = [na]
+    else:
+        rng2 = np.random.Generator(np.random.PCG64(base + 2))
+        R, _ = np.linalg.qr(rng2.standard_normal((na, na)))
+        Cb = np.asfortranarray(Ca @ R)
+        sys_['orbitals'] = [Ca, Cb]
+        sys_['occ'] = [np.full(na, 1.0), np.full(nb, 1.0)]
+        sys_['n_active'] = [na, nb]
+    if with_integrals:
+        ints = {}
+        for kind in ('W', 'V', 'WV'):
+            X2 = two_index(kind, edf, cdf)
+            X3 = three_index(kind, eao, cao, edf, cdf)
+            X3ri = three_index_ri(kind, eri, cri, eao, cao, edf, cdf) if kind != 'WV' else None
+            ints[kind] = (X3, X2, X3ri)
+        sys_['ints'] = ints
+    return sys_
+
+
+def random_system(n_ao, n_df, n_ri, n_occ, n_active=None, seed=11, uniform_orbitals=True):
+    """Well-conditioned random inputs in the style of the reference's unit tests:
+    uniform [0,1) three-index arrays, metrics U diag(1 + u) U^T with U = expm(skew)
+    (condition number <= 2; W and WV negative definite like a geminal), S positive definite."""
+    from scipy.linalg import expm
+    rng = np.random.Generator(np.random.PCG64(seed))
+    P = n_ao * (n_ao + 1) // 2
+
+    def rpd(n, sign=1.0):
+        X = rng.uniform(size=(n, n))
+        U = expm(0.5 * (X - X.T))
+        ev = 1.0 + rng.uniform(size=n)
+        M = sign * (U * ev) @ U.T
+        return np.asfortranarray(0.5 * (M + M.T))
+
+    ints = {}
+    for kind, sign in (('W', -1.0), ('V', 1.0), ('WV', -1.0)):
+        X3 = np.asfortranarray(rng.uniform(size=(P, n_df)))
+        X3ri = np.asfortranarray(rng.uniform(size=(n_ao * n_ri, n_df))) if kind != 'WV' else None
+        ints[kind] = (X3, rpd(n_df, sign), X3ri)
+    S = rpd(n_ao + n_ri)
+    n_spin = len(n_occ)
+    n_orb = max(n_occ) + 1
+    orbitals = [np.asfortranarray(rng.uniform(size=(n_ao, n_orb)) if uniform_orbitals
+                                  else rng.standard_normal((n_ao, n_orb))) for _ in range(n_spin)]
+    occ = [rng.uniform(0.5, 1.0, size=o) * (2.0 if n_spin == 1 else 1.0) for o in n_occ]
+    if n_active is None:
+        n_active = list(n_occ)
+    return dict(name='random', N=n_ao, Nd=n_df, Nr=n_ri, S=S, ints=ints, orbitals=orbitals, occ=occ,
+                n_active=list(n_active))

@@ -1,0 +1,86 @@
+"""FP64 GPU path vs the CPU oracle on identical inputs, through the C ABI.
+
+Tolerances:
+* well-conditioned random inputs (reference unit-test style, metric condition
+  number <= 2): every energy term and the Fock matrix to 1e-12 relative -- the
+  algebra of the GPU re-association is exact, so only rounding remains;
+* the s-Gaussian benchmark recipe: DF metrics keep eigenvalues down to ~2.5e-10
+  (condition ~1e11), so two valid FP64 evaluations (different eigensolvers /
+  1/lambda vs 1/sqrt|lambda| gauges) already differ by ~2e-10 Eh per term and
+  ~2e-9 in the Fock; the bar there is 2e-9 Eh per term and 2e-8 Fock max-abs.
+"""
+import numpy as np
+import pytest
+
+from helpers import TERMS, fock_error, fock_scale, gpu_eval, gpu_setup, oracle_eval
+from libuw12_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+RANDOM_CASES = [
+    # n_ao, n_df, n_ri, n_occ, n_active
+    (7, 10, 12, [4], [4]),
+    (7, 10, 12, [4, 3], [4, 3]),
+    (7, 10, 12, [4], [2]),
+    (7, 10, 12, [4, 3], [3, 1]),
+    (23, 57, 41, [9], [9]),
+    (23, 57, 41, [9, 6], [7, 6]),
+]
+
+
+@pytest.mark.parametrize('n_ao,n_df,n_ri,n_occ,n_active', RANDOM_CASES)
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0), (0.0, 1.0)])
+def test_random_well_conditioned(gpu_ctx, n_ao, n_df, n_ri, n_occ, n_active, kappa):
+    print(f'case random n_ao={n_ao} n_occ={n_occ} n_active={n_active} kappa={kappa}', flush=True)
+    sys_ = synth.random_system(n_ao, n_df, n_ri, n_occ, n_active, seed=n_ao * 31 + len(n_occ) + sum(n_active))
+    h = gpu_setup(sys_, gpu_ctx)
+    eg, Fg = gpu_eval(h, sys_, *kappa)
+    eo, Fo = oracle_eval(sys_, *kappa)
+    scale = max(abs(eo[k]) for k in TERMS)
+    for k in TERMS:
+        assert abs(eg[k] - eo[k]) <= 1e-12 * max(abs(eo[k]), 1e-3 * scale), (k, eg[k], eo[k])
+    assert abs(eg['total'] - eo['total']) <= 1e-12 * scale
+    assert fock_error(Fg, Fo) <= 1e-12 * fock_scale(Fo)
+
+
+@pytest.fixture(scope='module')
+def c1_system():
+    return synth.make_system('c1')
+
+
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0), (0.0, 1.0)])
+def test_c1_recipe(gpu_ctx, c1_system, kappa):
+    print(f'case c1 kappa={kappa}', flush=True)
+    h = gpu_setup(c1_system, gpu_ctx)
+    eg, Fg = gpu_eval(h, c1_system, *kappa)
+    eo, Fo = oracle_eval(c1_system, *kappa)
+    for k in TERMS + ('total',):
+        assert abs(eg[k] - eo[k]) <= 2e-9, (k, eg[k], eo[k])
+    assert fock_error(Fg, Fo) <= 2e-8
+
+
+def test_c1_frozen_core(gpu_ctx, c1_system):
+    print('case c1 frozen core n_active=4', flush=True)
+    h = gpu_setup(c1_system, gpu_ctx)
+    eg, Fg = gpu_eval(h, c1_system, 1.0, 0.5, n_active=[4])
+    eo, Fo = oracle_eval(c1_system, 1.0, 0.5, n_active=[4])
+    for k in TERMS + ('total',):
+        assert abs(eg[k] - eo[k]) <= 2e-9, (k, eg[k], eo[k])
+    assert fock_error(Fg, Fo) <= 2e-8
+
+
+def test_c1_open_shell(gpu_ctx, c1_system):
+    """Unrestricted c1: beta orbitals are a seeded rotation of the alpha set, one fewer occupied."""
+    print('case c1 open shell', flush=True)
+    Ca = c1_system['orbitals'][0]
+    rng = np.random.default_rng(1003)
+    R, _ = np.linalg.qr(rng.standard_normal((Ca.shape[1], Ca.shape[1])))
+    orbs = [Ca, np.asfortranarray(Ca @ R)]
+    occ = [np.ones(5), np.ones(4)]
+    act = [5, 4]
+    h = gpu_setup(c1_system, gpu_ctx)
+    eg, Fg = gpu_eval(h, c1_system, 1.0, 0.5, orbitals=orbs, occ=occ, n_active=act)
+    eo, Fo = oracle_eval(c1_system, 1.0, 0.5, orbitals=orbs, occ=occ, n_active=act)
+    for k in TERMS + ('total',):
+        assert abs(eg[k] - eo[k]) <= 2e-9, (k, eg[k], eo[k])
+    assert fock_error(Fg, Fo) <= 2e-8
