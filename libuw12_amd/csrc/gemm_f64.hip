@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <string>
 
 #include "uw12g_internal.h"
 
@@ -182,7 +183,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     }
   }
 
-  // ---- epilogue: lane holds C[m = .. + lr][n = .. + 4*lk + r]
+  // ---- epilogue.  v_mfma_f64_16x16x4f64 leaves D[i][j] of its 16x16 block in
+  // register i / 4 of lane j + 16 (i % 4) (one-row groups, four groups per
+  // block), so lane (lr, lk) register r holds C[m = .. + lr][n = .. + lk + 4 r].
   if (p.out) {
     double* W = p.out + (int64_t(blockIdx.y) * d.batch + b) * int64_t(d.M) * d.N;
 #pragma unroll
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       for (int j = 0; j < NB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn0 + 16 * j + 4 * lk + r;
+          const int n = n0 + wn0 + 16 * j + lk + 4 * r;
           if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * acc[i][j][r];
         }
     }
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn0 + 16 * j + 4 * lk + r;
+        const int n = n0 + wn0 + 16 * j + lk + 4 * r;
         if (n >= d.N) continue;
         double* c = Cp + mo + d.Cn.off(n);
         const double v = d.alpha * acc[i][j][r];
@@ -249,6 +252,12 @@ __global__ void gemm_naive_kernel(GemmDesc d) {
 }
 
 template <int BM, int BN>
+std::string cfg_name(bool akc, bool bkc) {
+  return "gemm_f64_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
+         (bkc ? "1" : "0") + ">";
+}
+
+template <int BM, int BN>
 void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
   if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), grid, dim3(NT), 0, s, p);
   else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), grid, dim3(NT), 0, s, p);
@@ -280,7 +289,7 @@ static void check_desc(const GemmDesc& d) {
   check_extent(d.C, d.Cb * (d.batch - 1) + map_max(d.Cm, d.M) + map_max(d.Cn, d.N), "gemm C");
 }
 
-void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws) {
+void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   check_desc(d);
   if (d.batch > 65535) throw Error(-2, "gemm batch exceeds 65535");
@@ -317,16 +326,33 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws) {
   if (splits > 1) p.out = ws->get(size_t(splits) * d.batch * size_t(d.M) * d.N, stream);
   p.remap = (tiles % 8 == 0 && tiles >= 64) ? 1 : 0;
   dim3 grid(unsigned(tiles), unsigned(splits), unsigned(d.batch));
-  if (BM == 128 && BN == 128) launch_cfg<128, 128>(p, akc, bkc, grid, stream);
-  else if (BM == 128) launch_cfg<128, 64>(p, akc, bkc, grid, stream);
-  else if (BN == 128) launch_cfg<64, 128>(p, akc, bkc, grid, stream);
-  else launch_cfg<64, 64>(p, akc, bkc, grid, stream);
+  const bool timing = prof && prof->on;
+  ProfileRec rec;
+  if (timing) prof->begin(rec, stream);
+  std::string kname;
+  if (BM == 128 && BN == 128) {
+    launch_cfg<128, 128>(p, akc, bkc, grid, stream);
+    if (timing) kname = cfg_name<128, 128>(akc, bkc);
+  } else if (BM == 128) {
+    launch_cfg<128, 64>(p, akc, bkc, grid, stream);
+    if (timing) kname = cfg_name<128, 64>(akc, bkc);
+  } else if (BN == 128) {
+    launch_cfg<64, 128>(p, akc, bkc, grid, stream);
+    if (timing) kname = cfg_name<64, 128>(akc, bkc);
+  } else {
+    launch_cfg<64, 64>(p, akc, bkc, grid, stream);
+    if (timing) kname = cfg_name<64, 64>(akc, bkc);
+  }
   UW12G_HIP(hipGetLastError());
+  if (timing) prof->end(rec, stream, kname, 2.0 * d.M * double(d.N) * d.K * d.batch, 0.0);
   if (splits > 1) {
     const int64_t MN = int64_t(d.M) * d.N;
     dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
+    ProfileRec r2;
+    if (timing) prof->begin(r2, stream);
     hipLaunchKernelGGL(splitk_reduce_kernel, g, dim3(256), 0, stream, d, p.out, splits);
     UW12G_HIP(hipGetLastError());
+    if (timing) prof->end(r2, stream, "splitk_reduce_kernel", double(splits) * d.batch * MN, 8.0 * (splits + 1) * d.batch * MN);
   }
 }
 

@@ -125,11 +125,27 @@ struct DBuf {
   }
 };
 
-struct ProfileRec {
-  std::string name;
-  hipEvent_t start, stop;
-  double flops, bytes;
-};
+void Profiler::begin(ProfileRec& r, hipStream_t s) {
+  UW12G_HIP(hipEventCreate(&r.start));
+  UW12G_HIP(hipEventCreate(&r.stop));
+  UW12G_HIP(hipEventRecord(r.start, s));
+}
+
+void Profiler::end(ProfileRec& r, hipStream_t s, const std::string& kernel, double flops, double bytes) {
+  UW12G_HIP(hipEventRecord(r.stop, s));
+  r.name = stage + "|" + kernel;
+  r.flops = flops;
+  r.bytes = bytes;
+  recs.push_back(r);
+}
+
+void Profiler::clear() {
+  for (auto& r : recs) {
+    if (r.start) (void)hipEventDestroy(r.start);
+    if (r.stop) (void)hipEventDestroy(r.stop);
+  }
+  recs.clear();
+}
 
 }  // namespace uw12g
 
@@ -141,9 +157,7 @@ struct uw12g_context {
   rocblas_handle rb = nullptr;
   Workspace gemm_ws;
   std::map<std::string, std::unique_ptr<DBuf>> bufs;
-  bool profiling = false;
-  std::vector<ProfileRec> prof;
-  std::vector<std::string> prof_names;  // stable storage for names handed out
+  Profiler prof;
 
   // Named device buffers reused across evaluations; growing or releasing one
   // first drains the stream so no queued kernel still reads the old memory.
@@ -160,10 +174,7 @@ struct uw12g_context {
     bufs.erase(it);
   }
   ~uw12g_context() {
-    for (auto& r : prof) {
-      (void)hipEventDestroy(r.start);
-      (void)hipEventDestroy(r.stop);
-    }
+    prof.clear();
     bufs.clear();
     if (rb) rocblas_destroy_handle(rb);
     if (stream) (void)hipStreamDestroy(stream);
@@ -209,41 +220,24 @@ void require(bool ok, const std::string& msg, int code = -1) {
 
 void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)); }
 
-// GEMM launch with optional per-launch timing.
-void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* name) {
+// GEMM launch (timed per launch when profiling is on).
+void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  ProfileRec rec;
-  if (ctx->profiling) {
-    UW12G_HIP(hipEventCreate(&rec.start));
-    UW12G_HIP(hipEventCreate(&rec.stop));
-    UW12G_HIP(hipEventRecord(rec.start, ctx->stream));
-  }
-  gemm_f64(d, ctx->stream, &ctx->gemm_ws);
-  if (ctx->profiling) {
-    UW12G_HIP(hipEventRecord(rec.stop, ctx->stream));
-    rec.name = name;
-    rec.flops = 2.0 * d.M * double(d.N) * d.K * d.batch;
-    rec.bytes = 0;
-    ctx->prof.push_back(rec);
-  }
+  ctx->prof.stage = stage;
+  gemm_f64(d, ctx->stream, &ctx->gemm_ws, &ctx->prof);
 }
 
 template <class F>
-void timed(uw12g_context* ctx, const char* name, double flops, double bytes, F&& f) {
-  if (!ctx->profiling) {
+void timed(uw12g_context* ctx, const char* stage, const char* kernel, double flops, double bytes, F&& f) {
+  if (!ctx->prof.on) {
     f();
     return;
   }
   ProfileRec rec;
-  UW12G_HIP(hipEventCreate(&rec.start));
-  UW12G_HIP(hipEventCreate(&rec.stop));
-  UW12G_HIP(hipEventRecord(rec.start, ctx->stream));
+  ctx->prof.stage = stage;
+  ctx->prof.begin(rec, ctx->stream);
   f();
-  UW12G_HIP(hipEventRecord(rec.stop, ctx->stream));
-  rec.name = name;
-  rec.flops = flops;
-  rec.bytes = bytes;
-  ctx->prof.push_back(rec);
+  ctx->prof.end(rec, ctx->stream, kernel, flops, bytes);
 }
 
 // FP64 symmetric eigendecomposition on the device (rocSOLVER syevd, lower
@@ -412,10 +406,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
   const bool calc_fock = fock != nullptr;
   const bool do_ind = indirect_term && k1 != 0.0;
   const double e = ns == 1 ? 2.0 : 1.0;
-  for (auto& r : ctx->prof) {
-    (void)hipEventDestroy(r.start);
-    (void)hipEventDestroy(r.stop);
-  }
   ctx->prof.clear();
 
   // ---- host: occupation-weighted orbitals and active block (utils.hpp:235-310)
@@ -482,7 +472,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     }
     // X_D[s]_A = sum_j (jj|A~)  (integrals.cpp:347-378); HBM-bound pass over Gamma_WV
     double* XD = ctx->buf("XD", size_t(std::max(DWV, 1)) * ns);
-    timed(ctx, "2e_XD_gemv", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
+    timed(ctx, "2e_XD", "gemv_t_kernel", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
       gemv_t(WV->gamma.p, int64_t(NN), int64_t(NN), DCa, int64_t(NN), ns, XD, DWV, DWV, st);
     });
     std::vector<double> XDh(size_t(std::max(DWV, 1)) * ns);
@@ -504,7 +494,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     if (calc_fock && DWV > 0) {
       double* vd = ctx->buf("XDv", size_t(DWV) * ns);
       UW12G_HIP(hipMemcpyAsync(vd, vh.data(), sizeof(double) * DWV * ns, hipMemcpyHostToDevice, st));
-      timed(ctx, "2e_fock_gemv", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
+      timed(ctx, "2e_fock", "gemv_n_kernel", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
         gemv_n(WV->gamma.p, int64_t(NN), int64_t(NN), vd, DWV, ns, Fraw, int64_t(NN), DWV, st);
       });
     }
@@ -820,7 +810,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           g.Cn = lin(ao);
           run_gemm(ctx, g, "4e_ind_x4");
         }
-        timed(ctx, "4e_ind_energy", 2.0 * ao * ao, 16.0 * ao * ao,
+        timed(ctx, "4e_ind_energy", "e4i_kernel", 2.0 * ao * ao, 16.0 * ao * ao,
               [&] { e4i_energy(W4, V4, a, o, -0.5 * e * k1, Eacc + UW12G_E4_INDIRECT, partial, st); });
         ctx->release("W4");
         ctx->release("V4");
@@ -948,7 +938,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           h.B = V2s + int64_t(a) * k0;
           h.C = GVb;
           run_gemm(ctx, h, "3e_ind_e_gv");
-          timed(ctx, "3e_ind_energy", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
+          timed(ctx, "3e_ind_energy", "e3i_kernel", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
                 [&] { e3i_energy(GWb, GVb, M, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
         }
         ctx->release("e3_gw");
@@ -1304,16 +1294,16 @@ int uw12g_fock(uw12g_context* ctx, double* fock, const uw12g_base_integrals* W, 
 int uw12g_context_set_profiling(uw12g_context* ctx, int on) {
   return guard([&] {
     require(ctx != nullptr, "null context");
-    ctx->profiling = on != 0;
+    ctx->prof.on = on != 0;
   });
 }
 
 int uw12g_context_profile(const uw12g_context* ctx, const char** names, double* ms, double* flops, double* bytes,
                           int cap) {
   if (!ctx) return -1;
-  const int n = int(ctx->prof.size());
+  const int n = int(ctx->prof.recs.size());
   for (int i = 0; i < n && i < cap; ++i) {
-    const ProfileRec& r = ctx->prof[i];
+    const ProfileRec& r = ctx->prof.recs[i];
     float t = 0.f;
     if (hipEventElapsedTime(&t, r.start, r.stop) != hipSuccess) t = -1.f;
     if (names) names[i] = r.name.c_str();

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 namespace uw12g {
 
@@ -94,8 +95,24 @@ void check_extent(const void* base, int64_t max_off, const char* what);
 // Largest offset a map produces over indices [0, extent).
 int64_t map_max(const Map2& m, int64_t extent);
 
+// Per-launch timing with HIP events on the launching stream (opt-in).
+struct ProfileRec {
+  std::string name;  // "stage|kernel"
+  hipEvent_t start = nullptr, stop = nullptr;
+  double flops = 0.0, bytes = 0.0;
+};
+struct Profiler {
+  bool on = false;
+  std::string stage;
+  std::vector<ProfileRec> recs;
+  void begin(ProfileRec& r, hipStream_t s);
+  void end(ProfileRec& r, hipStream_t s, const std::string& kernel, double flops, double bytes);
+  void clear();
+  ~Profiler() { clear(); }
+};
+
 struct Workspace;
-void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws);
+void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof = nullptr);
 void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
 
 // Device scratch owned by a context: grown on demand, reused across calls.

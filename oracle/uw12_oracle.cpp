@@ -55,6 +55,10 @@ dgemm_t g_dgemm = nullptr;
 dsyevd_t g_dsyevd = nullptr;
 bool g_blas_init = false;
 int g_force_builtin = 0;
+// Stride over the outermost same-spin pair index (1 = every pair).  Values > 1
+// evaluate a uniform sample of the pair loops; the CPU-baseline timing uses it
+// to bound its run time and extrapolates.
+int g_stride = 1;
 
 void blas_init() {
   if (g_blas_init) return;
@@ -66,7 +70,7 @@ void blas_init() {
     void* h = dlopen(n, RTLD_NOW | RTLD_LOCAL);
     if (!h) continue;
     auto set_layer = reinterpret_cast<int (*)(int)>(dlsym(h, "MKL_Set_Threading_Layer"));
-    if (set_layer) set_layer(1);  // MKL_THREADING_GNU: share libgomp with this file
+    if (set_layer) set_layer(3);  // MKL_THREADING_GNU (3): share libgomp with this file
     g_dgemm = reinterpret_cast<dgemm_t>(dlsym(h, "dgemm_"));
     g_dsyevd = reinterpret_cast<dsyevd_t>(dlsym(h, "dsyevd_"));
     if (g_dgemm && g_dsyevd) return;
@@ -509,7 +513,7 @@ void four_electron(const Ints& W, const Ints& V, bool indirect, bool calc_fock, 
     // indirect_energy, four_electron.cpp:143-179
     double acc = 0;
 #pragma omp parallel for reduction(+ : acc) collapse(2)
-    for (int k = 0; k < o; ++k)
+    for (int k = 0; k < o; k += g_stride)
       for (int l = 0; l < o; ++l) {
         double v = 0;
         for (int jj = 0; jj < a; ++jj)
@@ -530,7 +534,7 @@ void four_electron(const Ints& W, const Ints& V, bool indirect, bool calc_fock, 
     Mat fc(N, N);
     {
       const int ld3 = W3.r;
-      for (int i = 0; i < a; ++i)
+      for (int i = 0; i < a; i += g_stride)
         for (int k = 0; k < o; ++k) {
           // W4_ik = sub_mat(W3, (core+i)*N, a*k, N, a); V4_ik = reshape(tail(col(V3, i + a k), N a), N, a)
           const double* Wp = W3.d.data() + (size_t)(core + i) * N + (size_t)ld3 * (a * k);
@@ -541,7 +545,7 @@ void four_electron(const Ints& W, const Ints& V, bool indirect, bool calc_fock, 
     if (core != 0) {
       Mat fr(N, N);
       const int ld3 = W3.r;
-      for (int k = 0; k < o; ++k)
+      for (int k = 0; k < o; k += g_stride)
         for (int l = 0; l < o; ++l) {
           const double* Wp = W3.d.data() + (size_t)l * N + (size_t)ld3 * (k * a);
           const double* Vp = V3.d.data() + (size_t)k * N + (size_t)ld3 * (l * a);
@@ -685,7 +689,7 @@ double indirect_3el_energy(const Ints& W, const Ints& V, const ABS& P, int s) {
   {
     Mat x(NdW, NdV), t1(Nr, NdV), t2(N, NdV), Wt(o, NdW), Vt(o, NdV), ttij(NdW, NdV);
 #pragma omp for collapse(2) schedule(dynamic)
-    for (int i = 0; i < a; ++i)
+    for (int i = 0; i < a; i += g_stride)
       for (int j = 0; j < a; ++j) {
         // calculate_xij
         gemm('N', 'N', Nr, NdV, Nr, 1.0, P.rr.d.data(), Nr, Vr.d.data() + (size_t)j * Nr, Vr.r, 0.0, t1.d.data(), Nr);
@@ -722,7 +726,7 @@ Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
     Mat xt(NdX, a), xam(Nr * N, a), xap(PP, a), Yk(NdY, N), yvj(Nr, N), ypj(N, N);
     Mat t1(Nr, N), t2(N, N), sq(N, N);
 #pragma omp for schedule(dynamic)
-    for (int k = 0; k < o; ++k) {
+    for (int k = 0; k < o; k += g_stride) {
       for (int j = 0; j < a; ++j)
         for (int A = 0; A < NdX; ++A) xt(A, j) = X2(k * a + j, A) * Xb.vals[A];
       gemm('N', 'N', Nr * N, a, NdX, 1.0, Xb.J3ri.d.data(), Nr * N, xt.d.data(), NdX, 0.0, xam.d.data(), Nr * N);
@@ -762,7 +766,7 @@ Mat fock_k_sum(const Ints& W, const Ints& V, const ABS& P, int s) {
     Mat& f = part[omp_get_thread_num()];
     Mat Wj(NdW, N), Vi(NdV, N), Wjb_im(Nr, N), Vvj_ai(Nr, N), Vpj_ai(N, N), Wjb_ip(N, N), t1(Nr, N), t2(N, N);
 #pragma omp for collapse(2) schedule(dynamic)
-    for (int i = 0; i < a; ++i)
+    for (int i = 0; i < a; i += g_stride)
       for (int j = 0; j < a; ++j) {
         for (int b = 0; b < N; ++b) {
           for (int A = 0; A < NdW; ++A) Wj(A, b) = W.b->vals[A] * W1((j + core) * N + b, A);
@@ -805,6 +809,8 @@ thread_local std::string g_err;
 
 }  // namespace
 
+struct oracle_system;
+
 extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
@@ -822,26 +828,47 @@ void oracle_force_builtin_blas(int on) {
   g_dsyevd = nullptr;
 }
 
-// Full UW12 energy (+Fock) evaluation following the reference order.
+struct oracle_system {
+  Base W, V, WV;
+  ABS P;
+  int N = 0;
+};
+
+// Density-independent setup (the reference's setup_base_integrals x3 and
+// setup_abs_projectors, libuw12.cpp:22-56).
 //   X3*: packed (n_ao(n_ao+1)/2 x n_df) column-major, X2*: n_df x n_df,
 //   X3ri*: (n_ao n_ri x n_df) rows r + n_ri mu; S: (n_ao+n_ri)^2.
+int oracle_setup(const double* W3, const double* W2, const double* W3ri, const double* V3, const double* V2,
+                 const double* V3ri, const double* WV3, const double* WV2, const double* S, int n_ao, int n_df,
+                 int n_ri, oracle_system** out) {
+  try {
+    auto* sys = new oracle_system();
+    sys->N = n_ao;
+    base_setup(sys->W, W3, W2, W3ri, n_ao, n_df, n_ri);
+    base_setup(sys->V, V3, V2, V3ri, n_ao, n_df, n_ri);
+    base_setup(sys->WV, WV3, WV2, nullptr, n_ao, n_df, 0);
+    sys->P = abs_setup(S, n_ao, n_ri);
+    *out = sys;
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+void oracle_destroy(oracle_system* sys) { delete sys; }
+
+// form_fock (src/uw12.cpp:22-119) on a set-up system.
 //   C: n_ao x (n_orb n_spin) (alpha block then beta); occ: alpha then beta.
 //   terms_out[6]: e2 direct, e2 indirect, e3 direct, e3 indirect, e4 direct, e4 indirect.
-//   fock_out: n_ao x (n_ao n_spin) or NULL.
-int oracle_uw12(const double* W3, const double* W2, const double* W3ri, const double* V3,
-                const double* V2, const double* V3ri, const double* WV3, const double* WV2,
-                const double* S, int n_ao, int n_df, int n_ri, const double* C, int n_orb,
-                int n_spin, const double* occ, const int* n_occ, const int* n_active,
-                double k0, double k1, int indirect_term, double* fock_out, double* terms_out) {
+//   fock_out: n_ao x (n_ao n_spin) or NULL (energy only).
+int oracle_evaluate(const oracle_system* sys, const double* C, int n_orb, int n_spin, const double* occ,
+                    const int* n_occ, const int* n_active, double k0, double k1, int indirect_term,
+                    double* fock_out, double* terms_out) {
   try {
-    const int N = n_ao;
+    const int N = sys->N;
     const int ns = n_spin;
     if (ns != 1 && ns != 2) throw std::runtime_error("Number of spin channels must be 1 or 2");
-    Base W, V, WV;
-    base_setup(W, W3, W2, W3ri, N, n_df, n_ri);
-    base_setup(V, V3, V2, V3ri, N, n_df, n_ri);
-    base_setup(WV, WV3, WV2, nullptr, N, n_df, 0);
-    ABS P = abs_setup(S, N, n_ri);
     std::vector<Mat> Co(ns), Ca(ns);
     size_t occ_off = 0;
     for (int s = 0; s < ns; ++s) {  // occupation_weighted_orbitals + freeze_core
@@ -863,14 +890,14 @@ int oracle_uw12(const double* W3, const double* W2, const double* W3ri, const do
     const bool indirect = indirect_term != 0;
     Result R;
     R.F = zeros(ns, N);
-    two_electron(WV, Ca, indirect, calc_fock, k0, k1, R);
+    two_electron(sys->WV, Ca, indirect, calc_fock, k0, k1, R);
     Ints WI, VI;
-    ints_build(WI, W, Co, Ca, true);
-    ints_build(VI, V, Co, Ca, true);
+    ints_build(WI, sys->W, Co, Ca, true);
+    ints_build(VI, sys->V, Co, Ca, true);
     std::vector<Mat> tab = calc_tab(WI, VI);
     std::vector<Mat> tt = calc_ttilde(WI, VI, tab);
-    three_direct(WI, VI, P, calc_fock, k0, k1, R, tt);
-    if (indirect && k1 != 0) three_indirect(WI, VI, P, calc_fock, k1, R);
+    three_direct(WI, VI, sys->P, calc_fock, k0, k1, R, tt);
+    if (indirect && k1 != 0) three_indirect(WI, VI, sys->P, calc_fock, k1, R);
     four_electron(WI, VI, indirect, calc_fock, k0, k1, R, tab, tt);
     if (terms_out) {
       terms_out[0] = R.e2d;
@@ -888,6 +915,21 @@ int oracle_uw12(const double* W3, const double* W2, const double* W3ri, const do
     return -1;
   }
 }
+
+// Setup + evaluation in one call.
+int oracle_uw12(const double* W3, const double* W2, const double* W3ri, const double* V3, const double* V2,
+                const double* V3ri, const double* WV3, const double* WV2, const double* S, int n_ao, int n_df,
+                int n_ri, const double* C, int n_orb, int n_spin, const double* occ, const int* n_occ,
+                const int* n_active, double k0, double k1, int indirect_term, double* fock_out, double* terms_out) {
+  oracle_system* sys = nullptr;
+  int rc = oracle_setup(W3, W2, W3ri, V3, V2, V3ri, WV3, WV2, S, n_ao, n_df, n_ri, &sys);
+  if (rc != 0) return rc;
+  rc = oracle_evaluate(sys, C, n_orb, n_spin, occ, n_occ, n_active, k0, k1, indirect_term, fock_out, terms_out);
+  oracle_destroy(sys);
+  return rc;
+}
+
+void oracle_set_pair_stride(int stride) { g_stride = stride < 1 ? 1 : stride; }
 
 // The ABS+ projector blocks as one (n_ao+n_ri)^2 matrix, AO first.
 int oracle_abs_projector(const double* S, int n_ao, int n_ri, double* out) {

@@ -2,8 +2,11 @@
 
 Tolerances:
 * well-conditioned random inputs (reference unit-test style, metric condition
-  number <= 2): every energy term and the Fock matrix to 1e-12 relative -- the
-  algebra of the GPU re-association is exact, so only rounding remains;
+  number <= 2): every energy term to 1e-12 relative and the Fock matrix to
+  1e-10 of its largest element -- the algebra of the GPU re-association is
+  exact, so only rounding remains; Fock elements of ~1e6 are sums of
+  contributions several orders larger that cancel (same-spin i/j/k terms), so
+  their rounding floor sits above that of the energies;
 * the s-Gaussian benchmark recipe: DF metrics keep eigenvalues down to ~2.5e-10
   (condition ~1e11), so two valid FP64 evaluations (different eigensolvers /
   1/lambda vs 1/sqrt|lambda| gauges) already differ by ~2e-10 Eh per term and
@@ -40,7 +43,7 @@ def test_random_well_conditioned(gpu_ctx, n_ao, n_df, n_ri, n_occ, n_active, kap
     for k in TERMS:
         assert abs(eg[k] - eo[k]) <= 1e-12 * max(abs(eo[k]), 1e-3 * scale), (k, eg[k], eo[k])
     assert abs(eg['total'] - eo['total']) <= 1e-12 * scale
-    assert fock_error(Fg, Fo) <= 1e-12 * fock_scale(Fo)
+    assert fock_error(Fg, Fo) <= 1e-10 * fock_scale(Fo)
 
 
 @pytest.fixture(scope='module')
