@@ -37,20 +37,22 @@ inline double boys0(double t) {
 
 inline double norm(double a) { return std::pow(2.0 * a / kPi, 0.75); }
 
-inline double kernel(int kind, double p, double q, double R2) {
+// Bare two-centre operator integrals of unit-prefactor Gaussian charge
+// distributions exp(-p|r-P|^2), exp(-q|r-Q|^2), R2 = |P-Q|^2:
+//   kind 0: 1/r12; kind 1: exp(-g r12^2); kind 2: exp(-g r12^2)/r12.
+inline double bare(int kind, double p, double q, double R2, double g) {
   const double al = p * q / (p + q);
   const double pref = std::pow(kPi * kPi / (p * q), 1.5);
   if (kind == 0) return pref * 2.0 * std::sqrt(al / kPi) * boys0(al * R2);
+  const double ex = std::exp(-al * g / (al + g) * R2);
+  if (kind == 1) return pref * std::pow(al / (al + g), 1.5) * ex;
+  return pref * std::pow(al / kPi, 1.5) * ex * (2.0 * kPi / (al + g)) * boys0(al * al * R2 / (al + g));
+}
+
+inline double kernel(int kind, double p, double q, double R2) {
+  if (kind == 0) return bare(0, p, q, R2, 0.0);
   double out = 0.0;
-  for (int i = 0; i < g_gem.n; ++i) {
-    const double g = g_gem.g[i], c = g_gem.c[i];
-    const double ex = std::exp(-al * g / (al + g) * R2);
-    if (kind == 1) {
-      out -= c * pref * std::pow(al / (al + g), 1.5) * ex;
-    } else {
-      out -= c * pref * std::pow(al / kPi, 1.5) * ex * (2.0 * kPi / (al + g)) * boys0(al * al * R2 / (al + g));
-    }
-  }
+  for (int i = 0; i < g_gem.n; ++i) out -= g_gem.c[i] * bare(kind, p, q, R2, g_gem.g[i]);
   return out;
 }
 
@@ -137,6 +139,8 @@ void synth_three_index_ri(int kind, const double* eri, const double* cri, int Nr
         X3ri[row + rows * A] = pr.K * norm(edf[A]) * kernel(kind, pr.p, edf[A], dist2(pr.P, cdf + 3 * A));
     }
 }
+
+double synth_bare(int kind, double p, double q, double R2, double g) { return bare(kind, p, q, R2, g); }
 
 // Value of the operator's two-centre integral for two primitives (tests).
 double synth_pair_integral(int kind, double p, const double* P, double q, const double* Q) {

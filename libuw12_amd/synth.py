@@ -28,8 +28,12 @@ CONFIGS = {
     'c3': dict(atoms=100, box=30.0, min_sep=2.5, occ=(160, 150)),
     'c4': dict(atoms=100, box=30.0, min_sep=2.5, occ=(160, None)),
     'c5': dict(atoms=188, box=40.0, min_sep=2.5, occ=(300, 290), last=(4, 12, 16)),
+    # intermediate sizes for parity tests (same recipe and atom density as c2)
+    'm6': dict(atoms=6, box=10.6, min_sep=2.5, occ=(12, None)),
+    'm6u': dict(atoms=6, box=10.6, min_sep=2.5, occ=(12, 11)),
+    'm12': dict(atoms=12, box=13.4, min_sep=2.5, occ=(24, None)),
 }
-CONFIG_INDEX = {'c1': 1, 'c2': 2, 'c3': 3, 'c4': 4, 'c5': 5}
+CONFIG_INDEX = {'c1': 1, 'c2': 2, 'c3': 3, 'c4': 4, 'c5': 5, 'm6': 6, 'm6u': 7, 'm12': 8}
 
 
 def build():
@@ -52,6 +56,8 @@ def lib():
                                            ctypes.c_int, dp]
         L.synth_pair_integral.argtypes = [ctypes.c_int, ctypes.c_double, dp, ctypes.c_double, dp]
         L.synth_pair_integral.restype = ctypes.c_double
+        L.synth_bare.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+        L.synth_bare.restype = ctypes.c_double
         c = np.array(GEMINAL[0], dtype=np.float64)
         g = np.array(GEMINAL[1], dtype=np.float64)
         L.synth_set_geminal(len(c), c.ctypes.data_as(dp), g.ctypes.data_as(dp))
@@ -64,6 +70,12 @@ def _p(a):
 
 
 KINDS = {'V': 0, 'W': 1, 'WV': 2}
+
+
+def bare_integral(kind, p, q, R2, gamma=0.0):
+    """Two-centre integral of unit Gaussian distributions exp(-p r^2), exp(-q r^2) at
+    squared distance R2 under 1/r (kind 'V'), exp(-g r^2) ('W'), exp(-g r^2)/r ('WV')."""
+    return lib().synth_bare(KINDS[kind], float(p), float(q), float(R2), float(gamma))
 
 
 def geometry(n_atoms, box, min_sep, seed):

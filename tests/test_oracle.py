@@ -1,0 +1,214 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+* brute force: explicit four-index DF integrals and the documented index sums
+  (oracle/brute.py) agree with the oracle's reference-order evaluation per term;
+* calculus: the oracle Fock is the density derivative of its energy (central
+  finite differences along orbital paths, all-electron);
+* the reference test suite's relations (tests/test_uw12.cpp:44-71,
+  tests/multi_el_test_utils.hpp:25-311, tests/test_two_el.cpp:374-398):
+  linearity in the scale factors, W <-> V symmetry, open shell with equal
+  alpha/beta orbitals equals closed shell, no active orbitals gives zero,
+  energy-only equals the Fock path energy, indirect flag irrelevant at k1 = 0,
+  one-electron self-interaction cancellation;
+* the fallback BLAS/eigensolver path equals the MKL path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import TERMS
+from libuw12_amd import synth
+from oracle import brute, cpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def oracle(sys_, k0, k1, **kw):
+    return cpu.uw12(sys_['ints'], sys_['S'], kw.pop('orbitals', sys_['orbitals']), kw.pop('occ', sys_['occ']),
+                    kw.pop('n_active', sys_['n_active']), k0, k1, **kw)
+
+
+CASES = [([4], [4]), ([4, 3], [4, 3]), ([4], [2]), ([4, 3], [3, 1]), ([5, 2], [5, 0])]
+
+
+@pytest.mark.parametrize('n_occ,n_active', CASES)
+def test_oracle_matches_brute_force(n_occ, n_active):
+    sys_ = synth.random_system(7, 10, 12, n_occ, n_active, seed=5 + sum(n_occ))
+    eb = brute.energy_terms(sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], n_active, 1.0, 0.5)
+    eo, _ = oracle(sys_, 1.0, 0.5)
+    scale = max(abs(eb[k]) for k in TERMS)
+    for k in TERMS:
+        assert abs(eo[k] - eb[k]) <= 1e-12 * scale, (k, eo[k], eb[k])
+
+
+def test_oracle_matches_brute_force_on_recipe():
+    """c1 s-Gaussian recipe: the explicit metric inverse of the brute force loses
+    ~1e-7 relative to the metric's conditioning (~1e11), so this pins formulas only."""
+    sys_ = synth.make_system('c1')
+    eb = brute.energy_terms(sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], sys_['n_active'], 1.0, 0.5)
+    eo, _ = oracle(sys_, 1.0, 0.5)
+    for k in TERMS:
+        assert abs(eo[k] - eb[k]) <= 1e-6 * max(1.0, abs(eb[k])), (k, eo[k], eb[k])
+
+
+def _fd_check(sys_, k0, k1, h=1e-4, n_dir=2, seed=0):
+    rng = np.random.default_rng(seed)
+    n_spin = len(sys_['orbitals'])
+    g = 2.0 if n_spin == 1 else 1.0
+    e0, F = oracle(sys_, k0, k1)
+    worst = 0.0
+    for _ in range(n_dir):
+        dC = [rng.standard_normal(c.shape) for c in sys_['orbitals']]
+
+        def E(t):
+            C = [c + t * d for c, d in zip(sys_['orbitals'], dC)]
+            return oracle(sys_, k0, k1, orbitals=C, fock=False)[0]['total']
+
+        fd = (E(h) - E(-h)) / (2 * h)
+        an = 0.0
+        for s in range(n_spin):
+            o = len(sys_['occ'][s])
+            w = np.sqrt(np.asarray(sys_['occ'][s]) / g)
+            Co = sys_['orbitals'][s][:, :o] * w
+            dCo = dC[s][:, :o] * w
+            an += np.sum(F[s] * g * (dCo @ Co.T + Co @ dCo.T))
+        worst = max(worst, abs(fd - an) / abs(an))
+    return worst
+
+
+@pytest.mark.parametrize('n_occ', [[4], [4, 3]])
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
+def test_oracle_fock_is_energy_derivative(n_occ, kappa):
+    sys_ = synth.random_system(7, 10, 12, n_occ, None, seed=17)
+    assert _fd_check(sys_, *kappa) < 1e-6
+
+
+def test_oracle_fock_is_energy_derivative_on_recipe():
+    """On the c1 recipe the energy carries ~1e-10 relative rounding noise from the
+    metric conditioning, so the best central-difference step (1e-4) pins the Fock to ~1e-5."""
+    sys_ = synth.make_system('c1')
+    assert _fd_check(sys_, 1.0, 0.5, h=1e-4) < 3e-5
+
+
+@pytest.fixture(scope='module')
+def rsys():
+    return synth.random_system(8, 11, 13, [4, 3], [3, 2], seed=23)
+
+
+def test_linear_in_scale_factors(rsys):
+    e, F = oracle(rsys, 1.0, 0.5)
+    eos, Fos = oracle(rsys, 1.0, 0.0, indirect=True)
+    ess, Fss = oracle(rsys, 0.0, 1.0)
+    assert abs(eos['total'] + 0.5 * ess['total'] - e['total']) <= 1e-12 * abs(e['total'])
+    for s in range(2):
+        assert np.abs(Fos[s] + 0.5 * Fss[s] - F[s]).max() <= 1e-12 * np.abs(F[s]).max()
+
+
+def test_w_v_symmetry(rsys):
+    """Swapping W and V leaves the three- and four-electron terms unchanged (multi_el_test_utils.hpp:85-94)."""
+    sw = dict(rsys)
+    sw['ints'] = dict(rsys['ints'], W=rsys['ints']['V'], V=rsys['ints']['W'])
+    e1, F1 = oracle(rsys, 1.0, 0.5)
+    e2, F2 = oracle(sw, 1.0, 0.5)
+    for k in ('e3_direct', 'e3_indirect', 'e4_direct', 'e4_indirect'):
+        assert abs(e1[k] - e2[k]) <= 1e-12 * max(abs(e1[k]), 1.0), k
+
+
+def test_open_shell_equal_orbitals_is_closed_shell():
+    sys_ = synth.random_system(7, 10, 12, [4], None, seed=29)
+    C = sys_['orbitals'][0]
+    occ = sys_['occ'][0]
+    ec, Fc = oracle(sys_, 1.0, 0.5)
+    eo, Fo = oracle(sys_, 1.0, 0.5, orbitals=[C, C], occ=[occ / 2, occ / 2], n_active=[4, 4])
+    assert abs(ec['total'] - eo['total']) <= 1e-12 * abs(ec['total'])
+    for s in range(2):
+        assert np.abs(Fo[s] - Fc[0]).max() <= 1e-11 * np.abs(Fc[0]).max()
+
+
+def test_no_active_orbitals_gives_zero():
+    sys_ = synth.random_system(7, 10, 12, [4, 3], [0, 0], seed=31)
+    e, F = oracle(sys_, 1.0, 0.5)
+    assert all(abs(e[k]) == 0.0 for k in TERMS)
+    assert all(np.abs(f).max() == 0.0 for f in F)
+
+
+def test_energy_only_and_indirect_flag():
+    sys_ = synth.random_system(7, 10, 12, [4, 3], [3, 3], seed=37)
+    e1, _ = oracle(sys_, 1.0, 0.5)
+    e2, F2 = oracle(sys_, 1.0, 0.5, fock=False)
+    assert F2 is None
+    # the oracle's OpenMP reductions are order-free, so allow last-bit differences
+    assert abs(e1['total'] - e2['total']) <= 1e-13 * abs(e1['total'])
+    a, Fa = oracle(sys_, 1.0, 0.0, indirect=True)
+    b, Fb = oracle(sys_, 1.0, 0.0, indirect=False)
+    assert abs(a['total'] - b['total']) <= 1e-13 * abs(a['total'])
+    assert all(np.abs(x - y).max() <= 1e-13 * np.abs(x).max() for x, y in zip(Fa, Fb))
+
+
+def test_one_electron_two_electron_term_is_self_interaction_free():
+    """One alpha electron, no beta: the 2e direct and indirect terms cancel (test_two_el.cpp:374-398)."""
+    sys_ = synth.random_system(7, 10, 12, [1, 0], None, seed=41)
+    e, _ = oracle(sys_, 1.0, 1.0, occ=[np.ones(1), np.ones(0)], n_active=[1, 0])
+    assert abs(e['e2_direct'] + e['e2_indirect']) <= 1e-12 * abs(e['e2_direct'])
+    e2, _ = oracle(sys_, 1.0, 0.0, occ=[np.ones(1), np.ones(0)], n_active=[1, 0])
+    assert abs(e2['e2']) <= 1e-12 * abs(e['e2_direct'])
+
+
+def test_builtin_blas_matches_mkl():
+    if cpu.blas_kind() != 'mkl':
+        pytest.skip('MKL not present: the oracle already runs its built-in BLAS')
+    sys_ = synth.random_system(9, 12, 10, [4, 3], [3, 2], seed=43)
+    e1, F1 = oracle(sys_, 1.0, 0.5)
+    cpu.force_builtin_blas(True)
+    try:
+        e2, F2 = oracle(sys_, 1.0, 0.5)
+    finally:
+        cpu.force_builtin_blas(False)
+    scale = max(abs(e1[k]) for k in TERMS)
+    for k in TERMS:
+        assert abs(e1[k] - e2[k]) <= 1e-11 * scale, k
+    assert max(np.abs(a - b).max() for a, b in zip(F1, F2)) <= 1e-9 * max(np.abs(a).max() for a in F1)
+
+
+def test_recipe_conditioning_floor():
+    """Two valid FP64 evaluations of the c1 recipe (metric eigenvectors from MKL
+    vs the Jacobi fallback) differ by an amount set by the DF metric conditioning;
+    the GPU parity bar on this recipe (2e-9 Eh) sits above it."""
+    if cpu.blas_kind() != 'mkl':
+        pytest.skip('needs both eigensolvers')
+    sys_ = synth.make_system('c1')
+    e1, F1 = oracle(sys_, 1.0, 0.5)
+    cpu.force_builtin_blas(True)
+    try:
+        e2, F2 = oracle(sys_, 1.0, 0.5)
+    finally:
+        cpu.force_builtin_blas(False)
+    floor = max(abs(e1[k] - e2[k]) for k in TERMS)
+    assert floor < 2e-9
+    assert np.abs(F1[0] - F2[0]).max() < 2e-8
+
+
+@pytest.fixture(scope='module')
+def water():
+    return dict(np.load(os.path.join(HERE, 'golden', 'water.npz')))
+
+
+def water_system(w, zero_ri=True):
+    N = w['orbitals'].shape[0]
+    Nd = w['W2'].shape[0]
+    Nr = w['S'].shape[0] - N
+    ri = np.zeros((N * Nr, Nd), order='F')
+    ints = {'W': (w['W3'], w['W2'], ri), 'V': (w['V3'], w['V2'], ri), 'WV': (w['WV3'], w['WV2'], None)}
+    return dict(N=N, Nd=Nd, Nr=Nr, S=w['S'], ints=ints, orbitals=[w['orbitals']], occ=[w['occ'][:, 0]],
+                n_active=[int(w['n_active_neutral'][0])])
+
+
+def test_water_fixture_two_and_four_electron_terms(water):
+    """Reference water DF integrals (n_ao=24, n_df=116, negative-definite W2/WV2):
+    RI-free terms against the brute force, frozen core n_active=4."""
+    sys_ = water_system(water)
+    eo, _ = oracle(sys_, 1.0, 0.5)
+    eb = brute.energy_terms(sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], sys_['n_active'], 1.0, 0.5)
+    for k in ('e2_direct', 'e2_indirect', 'e4_direct', 'e4_indirect'):
+        assert abs(eo[k] - eb[k]) <= 1e-9 * max(1e-3, abs(eb[k])), (k, eo[k], eb[k])
