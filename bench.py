@@ -68,6 +68,15 @@ def roofline_from_profile(records):
     table = {k: dict(ms_total=round(v[0], 3), launches=v[3], avg_us=round(1e3 * v[0] / v[3], 2),
                      tflops=round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[1] and v[0] > 0 else None)
              for k, v in sorted(agg.items(), key=lambda kv: -kv[1][0])}
+    stages = {}
+    for name, ms_, fl_, by_ in records:
+        st = stages.setdefault(name, [0.0, 0.0, 0])
+        st[0] += ms_
+        st[1] += fl_
+        st[2] += 1
+    table['_by_stage'] = {k: dict(ms_total=round(v[0], 3), launches=v[2],
+                                  tflops=round(v[1] / (v[0] * 1e-3) / 1e12, 2) if v[1] and v[0] > 0 else None)
+                          for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])}
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
         roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit='TFLOP/s',
