@@ -57,5 +57,19 @@ def build(verbose=False):
     return OUT
 
 
+def build_cpp_driver():
+    """C++ caller of include/uw12g.hpp used by the C++ API test (tests/cpp)."""
+    root = os.path.dirname(HERE)
+    out = os.path.join(root, 'tests', 'cpp', 'uw12_cpp_driver')
+    cmd = ['g++', '-O2', '-std=c++17', '-I', os.path.join(root, 'include'),
+           os.path.join(root, 'tests', 'cpp', 'uw12_cpp_driver.cpp'), '-o', out, f'-L{HERE}', '-luw12g',
+           f'-Wl,-rpath,{HERE}', f'-Wl,-rpath,{ROCM}/lib']
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'C++ driver build failed:\n{r.stderr}')
+    return out
+
+
 if __name__ == '__main__':
     print(build(verbose=True))
+    print(build_cpp_driver())
