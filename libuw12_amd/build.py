@@ -21,6 +21,8 @@ COMMON = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
     # keep hipcc from emitting packed FP32 instructions anywhere in this library
     '-Xclang', '-target-feature', '-Xclang', '-packed-fp32-ops',
+    # MFMA accumulators in ordinary VGPRs: no per-iteration AGPR copies, 2 waves/SIMD at 128x128 tiles
+    '-mllvm', '-amdgpu-mfma-vgpr-form',
     '-Wall', '-Wno-unused-result', '-Wno-unused-variable',
 ]
 

@@ -31,23 +31,27 @@ struct Params {
   int remap;    // 1: spread consecutive tiles over XCDs in contiguous chunks
 };
 
-template <int BM, int BN, bool AKC, bool BKC>
+// One 16-deep K slab per iteration, double-buffered in LDS (one barrier per
+// slab).  Global loads for slab t+1 are issued before the MFMAs of slab t and
+// consumed only when written to LDS after them, so their latency overlaps the
+// matrix work: every load address is clamped to an in-bounds element (offset 0)
+// when out of range and the predicate is applied at the LDS write.
+template <int BM, int BN, bool AKC, bool BKC, bool KS>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   constexpr int PAD = 4;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
-  constexpr int NA = BM * BK / NT;  // A elements loaded per thread
+  constexpr int NA = BM * BK / NT;  // A elements loaded per thread and slab
   constexpr int NBL = BN * BK / NT;
-  __shared__ double As[BK][BM + PAD];
-  __shared__ double Bs[BK][BN + PAD];
+  __shared__ double As[2][BK][BM + PAD];
+  __shared__ double Bs[2][BK][BN + PAD];
 
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int n_tiles = p.tiles_m * p.tiles_n;
   int tile = blockIdx.x;
   if (p.remap) {  // blockIdx -> XCD is round-robin over 8 XCDs; give each XCD a contiguous range
-    const int per = n_tiles >> 3;
+    const int per = (p.tiles_m * p.tiles_n) >> 3;
     tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
   const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
@@ -56,67 +60,59 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const int k_begin = blockIdx.y * p.kchunk;
   const int k_end = min(d.K, k_begin + p.kchunk);
 
-  const double* Ap = d.A + d.Ab * b;
-  const double* Bp = d.B + d.Bb * b;
-  const double* Kp = d.ks ? d.ks + d.Kb * b : nullptr;
+  const double* __restrict__ Ap = d.A + d.Ab * b;
+  const double* __restrict__ Bp = d.B + d.Bb * b;
+  const double* __restrict__ Kp = KS ? d.ks + d.Kb * b : nullptr;
 
-  // ---- per-thread load assignment
-  // A: AKC -> kk fixed = t % BK, rows t/BK + (NT/BK) i ; else mm fixed = t % BM, kk = t/BM + (NT/BM) i
-  int64_t a_moff[AKC ? NA : 1];
-  bool a_mok[AKC ? NA : 1];
-  int a_kk0;
-  if constexpr (AKC) {
-    a_kk0 = t % BK;
+  // ---- per-thread load assignment (fixed over the K loop)
+  // A: AKC -> k = t % BK, rows t/BK + (NT/BK) i ; else row t % BM, k = t/BM + (NT/BM) i
+  constexpr int NAM = AKC ? NA : 1;
+  int64_t a_moff[NAM];
+  uint32_t a_mok = 0;
+  const int a_kk0 = AKC ? t % BK : t / BM;
 #pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      const int m = m0 + t / BK + (NT / BK) * i;
-      a_mok[i] = m < d.M;
-      a_moff[i] = a_mok[i] ? d.Am.off(m) : 0;
-    }
-  } else {
-    a_kk0 = t / BM;
-    const int m = m0 + t % BM;
-    a_mok[0] = m < d.M;
-    a_moff[0] = a_mok[0] ? d.Am.off(m) : 0;
+  for (int i = 0; i < NAM; ++i) {
+    const int m = m0 + (AKC ? t / BK + (NT / BK) * i : t % BM);
+    const bool ok = m < d.M;
+    a_mok |= uint32_t(ok) << i;
+    a_moff[i] = ok ? d.Am.off(m) : 0;
   }
-  int64_t b_noff[BKC ? NBL : 1];
-  bool b_nok[BKC ? NBL : 1];
-  int b_kk0;
-  if constexpr (BKC) {
-    b_kk0 = t % BK;
+  constexpr int NBN = BKC ? NBL : 1;
+  int64_t b_noff[NBN];
+  uint32_t b_nok = 0;
+  const int b_kk0 = BKC ? t % BK : t / BN;
 #pragma unroll
-    for (int i = 0; i < NBL; ++i) {
-      const int n = n0 + t / BK + (NT / BK) * i;
-      b_nok[i] = n < d.N;
-      b_noff[i] = b_nok[i] ? d.Bn.off(n) : 0;
-    }
-  } else {
-    b_kk0 = t / BN;
-    const int n = n0 + t % BN;
-    b_nok[0] = n < d.N;
-    b_noff[0] = b_nok[0] ? d.Bn.off(n) : 0;
+  for (int i = 0; i < NBN; ++i) {
+    const int n = n0 + (BKC ? t / BK + (NT / BK) * i : t % BN);
+    const bool ok = n < d.N;
+    b_nok |= uint32_t(ok) << i;
+    b_noff[i] = ok ? d.Bn.off(n) : 0;
   }
 
-  double ra[NA], rb[NBL];
+  double ra[NA], rb[NBL], rs[AKC ? 1 : NA];
+  uint32_t a_ok = 0, b_ok = 0;
   auto load = [&](int k0) {
+    a_ok = 0;
+    b_ok = 0;
     if constexpr (AKC) {
       const int k = k0 + a_kk0;
       const bool kok = k < k_end;
       const int64_t ko = kok ? d.Ak.off(k) : 0;
-      const double sc = (Kp && kok) ? Kp[d.Km.off(k)] : 1.0;
+      if constexpr (KS) rs[0] = Kp[kok ? d.Km.off(k) : 0];
 #pragma unroll
-      for (int i = 0; i < NA; ++i) ra[i] = (kok && a_mok[i]) ? Ap[a_moff[i] + ko] * sc : 0.0;
+      for (int i = 0; i < NA; ++i) {
+        const bool ok = kok && ((a_mok >> i) & 1u);
+        a_ok |= uint32_t(ok) << i;
+        ra[i] = Ap[ok ? a_moff[i] + ko : 0];
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
         const int k = k0 + a_kk0 + (NT / BM) * i;
-        const bool ok = k < k_end && a_mok[0];
-        double v = 0.0;
-        if (ok) {
-          v = Ap[a_moff[0] + d.Ak.off(k)];
-          if (Kp) v *= Kp[d.Km.off(k)];
-        }
-        ra[i] = v;
+        const bool ok = k < k_end && (a_mok & 1u);
+        a_ok |= uint32_t(ok) << i;
+        ra[i] = Ap[ok ? a_moff[0] + d.Ak.off(k) : 0];
+        if constexpr (KS) rs[i] = Kp[ok ? d.Km.off(k) : 0];
       }
     }
     if constexpr (BKC) {
@@ -124,30 +120,34 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       const bool kok = k < k_end;
       const int64_t ko = kok ? d.Bk.off(k) : 0;
 #pragma unroll
-      for (int i = 0; i < NBL; ++i) rb[i] = (kok && b_nok[i]) ? Bp[b_noff[i] + ko] : 0.0;
+      for (int i = 0; i < NBL; ++i) {
+        const bool ok = kok && ((b_nok >> i) & 1u);
+        b_ok |= uint32_t(ok) << i;
+        rb[i] = Bp[ok ? b_noff[i] + ko : 0];
+      }
     } else {
 #pragma unroll
       for (int i = 0; i < NBL; ++i) {
         const int k = k0 + b_kk0 + (NT / BN) * i;
-        const bool ok = k < k_end && b_nok[0];
-        rb[i] = ok ? Bp[b_noff[0] + d.Bk.off(k)] : 0.0;
+        const bool ok = k < k_end && (b_nok & 1u);
+        b_ok |= uint32_t(ok) << i;
+        rb[i] = Bp[ok ? b_noff[0] + d.Bk.off(k) : 0];
       }
     }
   };
-  auto store = [&]() {
-    if constexpr (AKC) {
+  auto store = [&](int buf) {
 #pragma unroll
-      for (int i = 0; i < NA; ++i) As[a_kk0][t / BK + (NT / BK) * i] = ra[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < NA; ++i) As[a_kk0 + (NT / BM) * i][t % BM] = ra[i];
+    for (int i = 0; i < NA; ++i) {
+      double v = ((a_ok >> i) & 1u) ? ra[i] : 0.0;
+      if constexpr (KS) v *= rs[AKC ? 0 : i];
+      if constexpr (AKC) As[buf][a_kk0][t / BK + (NT / BK) * i] = v;
+      else As[buf][a_kk0 + (NT / BM) * i][t % BM] = v;
     }
-    if constexpr (BKC) {
 #pragma unroll
-      for (int i = 0; i < NBL; ++i) Bs[b_kk0][t / BK + (NT / BK) * i] = rb[i];
-    } else {
-#pragma unroll
-      for (int i = 0; i < NBL; ++i) Bs[b_kk0 + (NT / BN) * i][t % BN] = rb[i];
+    for (int i = 0; i < NBL; ++i) {
+      const double v = ((b_ok >> i) & 1u) ? rb[i] : 0.0;
+      if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = v;
+      else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = v;
     }
   };
 
@@ -162,24 +162,25 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
 
   if (k_begin < k_end) {
     load(k_begin);
+    int buf = 0;
     for (int k0 = k_begin; k0 < k_end; k0 += BK) {
-      __syncthreads();
-      store();
+      store(buf);
       __syncthreads();
       if (k0 + BK < k_end) load(k0 + BK);
 #pragma unroll
       for (int ks = 0; ks < BK; ks += 4) {
         double af[MB], bf[NB];
 #pragma unroll
-        for (int i = 0; i < MB; ++i) af[i] = As[ks + lk][wm0 + 16 * i + lr];
+        for (int i = 0; i < MB; ++i) af[i] = As[buf][ks + lk][wm0 + 16 * i + lr];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) bf[j] = Bs[ks + lk][wn0 + 16 * j + lr];
+        for (int j = 0; j < NB; ++j) bf[j] = Bs[buf][ks + lk][wn0 + 16 * j + lr];
 #pragma unroll
         for (int i = 0; i < MB; ++i)
 #pragma unroll
           for (int j = 0; j < NB; ++j)
             acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
       }
+      buf ^= 1;
     }
   }
 
@@ -252,28 +253,33 @@ __global__ void gemm_naive_kernel(GemmDesc d) {
 }
 
 template <int BM, int BN>
-std::string cfg_name(bool akc, bool bkc) {
+std::string cfg_name(bool akc, bool bkc, bool ks) {
   return "gemm_f64_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
-         (bkc ? "1" : "0") + ">";
+         (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + ">";
+}
+
+template <int BM, int BN, bool KS>
+void launch_ks(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
+  if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, KS>), grid, dim3(NT), 0, s, p);
+  else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, KS>), grid, dim3(NT), 0, s, p);
+  else if (bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, KS>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, KS>), grid, dim3(NT), 0, s, p);
 }
 
 template <int BM, int BN>
 void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
-  if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true>), grid, dim3(NT), 0, s, p);
-  else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false>), grid, dim3(NT), 0, s, p);
-  else if (bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true>), grid, dim3(NT), 0, s, p);
-  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false>), grid, dim3(NT), 0, s, p);
+  if (p.d.ks) launch_ks<BM, BN, true>(p, akc, bkc, grid, s);
+  else launch_ks<BM, BN, false>(p, akc, bkc, grid, s);
 }
 
 // Unit stride on the low level of a map (contiguous runs along that index).
-bool unit(const Map2& m, int extent) { return m.s0 == 1 && (m.n0 == 0 || m.n0 >= 2 || extent <= 1); }
+bool unit(const Map2& m) { return m.s0 == 1 && m.n0 >= 2; }
 
 }  // namespace
 
 int64_t map_max(const Map2& m, int64_t extent) {
   if (extent <= 0) return 0;
   const int64_t last = extent - 1;
-  if (m.n0 == 0) return last * m.s0;
   const int64_t q = last / m.n0, r = last % m.n0;
   int64_t best = r * m.s0 + q * m.s1;
   if (q >= 1) best = std::max(best, int64_t(m.n0 - 1) * m.s0 + (q - 1) * m.s1);
@@ -302,8 +308,9 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   // Tile shape: large tiles for large outputs, small for thin ones.
   int BM = d.M >= 96 ? 128 : 64;
   int BN = d.N >= 96 ? 128 : 64;
-  const bool akc = unit(d.Ak, d.K) && !(unit(d.Am, d.M) && d.Am.n0 == 0 && d.Ak.s0 != 1);
-  const bool bkc = unit(d.Bk, d.K);
+  // coalesce along k only when that index has the unit stride and the other does not
+  const bool akc = unit(d.Ak) && !unit(d.Am);
+  const bool bkc = unit(d.Bk) && !unit(d.Bn);
   Params p;
   p.d = d;
   p.tiles_m = (d.M + BM - 1) / BM;
@@ -332,16 +339,16 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   std::string kname;
   if (BM == 128 && BN == 128) {
     launch_cfg<128, 128>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<128, 128>(akc, bkc);
+    if (timing) kname = cfg_name<128, 128>(akc, bkc, d.ks != nullptr);
   } else if (BM == 128) {
     launch_cfg<128, 64>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<128, 64>(akc, bkc);
+    if (timing) kname = cfg_name<128, 64>(akc, bkc, d.ks != nullptr);
   } else if (BN == 128) {
     launch_cfg<64, 128>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<64, 128>(akc, bkc);
+    if (timing) kname = cfg_name<64, 128>(akc, bkc, d.ks != nullptr);
   } else {
     launch_cfg<64, 64>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<64, 64>(akc, bkc);
+    if (timing) kname = cfg_name<64, 64>(akc, bkc, d.ks != nullptr);
   }
   UW12G_HIP(hipGetLastError());
   if (timing) prof->end(rec, stream, kname, 2.0 * d.M * double(d.N) * d.K * d.batch, 0.0);

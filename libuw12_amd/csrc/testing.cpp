@@ -13,7 +13,7 @@ void set_last_error(const std::string& msg);
 using namespace uw12g;
 
 namespace {
-Map2 to_map(const uw12g_map2& m) { return m.n0 == 0 ? lin(m.s0) : two(m.n0, m.s0, m.s1); }
+Map2 to_map(const uw12g_map2& m) { return m.n0 == 0 ? lin(m.s0) : two(m.n0, m.s0, m.s1); }  // n0 == 0: one level
 
 struct Dev {
   double* p = nullptr;

@@ -37,14 +37,14 @@ __host__ __device__ inline uint32_t fd_div(uint32_t x, const FastDiv& f) {
   return uint32_t((uint64_t(hi) + x) >> f.s);
 }
 
-// Two-level index map: off(i) = (i mod n0) s0 + (i div n0) s1.
-// n0 == 0 marks a plain one-level map off(i) = i s0.
+// Two-level index map: off(i) = (i mod n0) s0 + (i div n0) s1.  A one-level
+// map off(i) = i s0 uses n0 = 2^32 - 1 (the quotient is 0 for every valid i),
+// so evaluation is branch-free on the device.
 struct Map2 {
   int64_t s0 = 0, s1 = 0;
-  uint32_t n0 = 0;
-  FastDiv fd;
+  uint32_t n0 = 0xffffffffu;
+  FastDiv fd = {0xffffffffu, 2u, 32u};
   __host__ __device__ inline int64_t off(uint32_t i) const {
-    if (n0 == 0) return int64_t(i) * s0;
     const uint32_t q = fd_div(i, fd);
     return int64_t(i - q * n0) * s0 + int64_t(q) * s1;
   }

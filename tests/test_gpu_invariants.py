@@ -144,8 +144,9 @@ def test_abs_projector_matches_oracle(gpu_ctx):
     P = u.setup_abs_projectors(s['S'], s['N'], s['Nr'], context=gpu_ctx)
     Sg = P.matrix()
     So = cpu.abs_projector(s['S'], s['N'], s['Nr'])
-    # S+ has norm ~1e6 here (kept eigenvalues down to ~1e-6); compare relative to it
-    assert np.abs(Sg - So).max() <= 1e-9 * np.abs(So).max()
+    # S+ has norm ~2e5 here (kept eigenvalues down to ~1e-6): two FP64 eigensolvers
+    # differ by ~|S+|^2 eps |S| in it, ~2e-9 relative on this recipe
+    assert np.abs(Sg - So).max() <= 1e-8 * np.abs(So).max()
 
 
 @pytest.mark.parametrize('name', ['m6', 'm6u'])
