@@ -201,6 +201,8 @@ def test_water_integrals(gpu_ctx):
         h = gpu_setup(s, gpu_ctx)
         eg, Fg = gpu_eval(h, s, 1.0, 0.5)
         eo, Fo = oracle_eval(s, 1.0, 0.5)
+        # with the RI block zeroed the three-electron term is unphysically large
+        # (~2e5 Eh); the bar is relative to each term's size
         for k in TERMS + ('total',):
-            assert abs(eg[k] - eo[k]) <= 1e-9, (spin, k, eg[k], eo[k])
-        assert fock_error(Fg, Fo) <= 1e-8
+            assert abs(eg[k] - eo[k]) <= 1e-9 * max(1.0, abs(eo[k])), (spin, k, eg[k], eo[k])
+        assert fock_error(Fg, Fo) <= 1e-8 * max(1.0, max(np.abs(f).max() for f in Fo))
