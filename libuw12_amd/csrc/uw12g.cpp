@@ -40,27 +40,33 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 namespace {
-std::mutex g_alloc_mu;
+// Never destroyed: contexts held in static storage of a caller are destroyed
+// at process exit in an order this library does not control, and their buffers
+// unregister here.
+std::mutex& alloc_mu() {
+  static std::mutex* mu = new std::mutex();
+  return *mu;
+}
 std::map<uintptr_t, size_t>& allocs() {
-  static std::map<uintptr_t, size_t> m;
-  return m;
+  static std::map<uintptr_t, size_t>* m = new std::map<uintptr_t, size_t>();
+  return *m;
 }
 }  // namespace
 
 void register_alloc(const void* p, size_t bytes) {
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  std::lock_guard<std::mutex> lk(alloc_mu());
   allocs()[reinterpret_cast<uintptr_t>(p)] = bytes;
 }
 
 void unregister_alloc(const void* p) {
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  std::lock_guard<std::mutex> lk(alloc_mu());
   allocs().erase(reinterpret_cast<uintptr_t>(p));
 }
 
 void check_extent(const void* base, int64_t max_off, const char* what) {
   if (!base) throw Error(-2, std::string(what) + ": null operand");
   if (max_off < 0) throw Error(-2, std::string(what) + ": negative extent");
-  std::lock_guard<std::mutex> lk(g_alloc_mu);
+  std::lock_guard<std::mutex> lk(alloc_mu());
   const uintptr_t b = reinterpret_cast<uintptr_t>(base);
   auto& m = allocs();
   auto it = m.upper_bound(b);
