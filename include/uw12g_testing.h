@@ -27,6 +27,13 @@ int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, do
                     uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,
                     uw12g_map2 Cn, long long Cb, const double* ks, size_t nks, uw12g_map2 Km, int naive);
 
+/* Average time in milliseconds of `reps` launches of the same GEMM on arrays
+ * already in device memory (after one untimed launch), on a stream of its own. */
+int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
+                    uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, uw12g_map2 Bk, uw12g_map2 Bn,
+                    long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks,
+                    uw12g_map2 Km, int reps, double* ms);
+
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);
 

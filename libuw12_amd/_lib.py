@@ -23,7 +23,7 @@ EXPORTS = (
     'uw12g_setup_base_integrals_device', 'uw12g_base_integrals_destroy', 'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
     'uw12g_fock', 'uw12g_form_fock', 'uw12g_context_set_profiling', 'uw12g_context_profile',
-    'uw12g_test_gemm', 'uw12g_test_map2',
+    'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm',
 )
 
 
@@ -67,6 +67,12 @@ def load():
                                   dp, ctypes.c_size_t, Map2, Map2, ctypes.c_longlong,
                                   dp, ctypes.c_size_t, Map2, Map2, ctypes.c_longlong,
                                   dp, ctypes.c_size_t, Map2, ctypes.c_int]
+    L.uw12g_time_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double,
+                                  vp, Map2, Map2, ctypes.c_longlong,
+                                  vp, Map2, Map2, ctypes.c_longlong,
+                                  vp, Map2, Map2, ctypes.c_longlong,
+                                  vp, Map2, ctypes.c_int, dp]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]
     L.uw12g_test_map2.restype = ctypes.c_longlong
     _lib = L

@@ -36,9 +36,12 @@ struct Params {
 // consumed only when written to LDS after them, so their latency overlaps the
 // matrix work: every load address is clamped to an in-bounds element (offset 0)
 // when out of range and the predicate is applied at the LDS write.
+#ifndef UW12G_LDS_PAD
+#define UW12G_LDS_PAD 4
+#endif
 template <int BM, int BN, bool AKC, bool BKC, bool KS>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
-  constexpr int PAD = 4;
+  constexpr int PAD = UW12G_LDS_PAD;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int NA = BM * BK / NT;  // A elements loaded per thread and slab
@@ -65,11 +68,14 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const double* __restrict__ Kp = KS ? d.ks + d.Kb * b : nullptr;
 
   // ---- per-thread load assignment (fixed over the K loop)
-  // A: AKC -> k = t % BK, rows t/BK + (NT/BK) i ; else row t % BM, k = t/BM + (NT/BM) i
+  // A: AKC -> k = t % BK, rows t/BK + (NT/BK) i ; else row t % BM, k = t/BM + (NT/BM) i.
+  // In the second form every k a wave loads is wave-uniform (BM, BN >= 64), so
+  // its offset, range test and scale factor are computed once per wave on the
+  // scalar unit and the vector work per element is one address add.
   constexpr int NAM = AKC ? NA : 1;
   int64_t a_moff[NAM];
   uint32_t a_mok = 0;
-  const int a_kk0 = AKC ? t % BK : t / BM;
+  const int a_kk0 = AKC ? t % BK : __builtin_amdgcn_readfirstlane(t / BM);
 #pragma unroll
   for (int i = 0; i < NAM; ++i) {
     const int m = m0 + (AKC ? t / BK + (NT / BK) * i : t % BM);
@@ -80,7 +86,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   constexpr int NBN = BKC ? NBL : 1;
   int64_t b_noff[NBN];
   uint32_t b_nok = 0;
-  const int b_kk0 = BKC ? t % BK : t / BN;
+  const int b_kk0 = BKC ? t % BK : __builtin_amdgcn_readfirstlane(t / BN);
 #pragma unroll
   for (int i = 0; i < NBN; ++i) {
     const int n = n0 + (BKC ? t / BK + (NT / BK) * i : t % BN);
@@ -106,13 +112,14 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
         ra[i] = Ap[ok ? a_moff[i] + ko : 0];
       }
     } else {
+      // row m = 0 and column k = 0 exist, so a clamped offset stays in bounds
 #pragma unroll
       for (int i = 0; i < NA; ++i) {
-        const int k = k0 + a_kk0 + (NT / BM) * i;
-        const bool ok = k < k_end && (a_mok & 1u);
-        a_ok |= uint32_t(ok) << i;
-        ra[i] = Ap[ok ? a_moff[0] + d.Ak.off(k) : 0];
-        if constexpr (KS) rs[i] = Kp[ok ? d.Km.off(k) : 0];
+        const int k = k0 + a_kk0 + (NT / BM) * i;  // wave-uniform
+        const bool kok = k < k_end;
+        a_ok |= uint32_t(kok) << i;
+        ra[i] = Ap[a_moff[0] + (kok ? d.Ak.off(k) : 0)];
+        if constexpr (KS) rs[i] = Kp[kok ? d.Km.off(k) : 0];
       }
     }
     if constexpr (BKC) {
@@ -128,24 +135,28 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     } else {
 #pragma unroll
       for (int i = 0; i < NBL; ++i) {
-        const int k = k0 + b_kk0 + (NT / BN) * i;
-        const bool ok = k < k_end && (b_nok & 1u);
-        b_ok |= uint32_t(ok) << i;
-        rb[i] = Bp[ok ? b_noff[0] + d.Bk.off(k) : 0];
+        const int k = k0 + b_kk0 + (NT / BN) * i;  // wave-uniform
+        const bool kok = k < k_end;
+        b_ok |= uint32_t(kok) << i;
+        rb[i] = Bp[b_noff[0] + (kok ? d.Bk.off(k) : 0)];
       }
     }
   };
+  // non-k-contiguous forms: a_ok / b_ok hold the uniform k tests, a_mok / b_nok the lane's row test
+  const uint32_t a_lane = AKC ? ~0u : (a_mok & 1u ? ~0u : 0u);
+  const uint32_t b_lane = BKC ? ~0u : (b_nok & 1u ? ~0u : 0u);
   auto store = [&](int buf) {
+    const uint32_t am = a_ok & a_lane, bm = b_ok & b_lane;
 #pragma unroll
     for (int i = 0; i < NA; ++i) {
-      double v = ((a_ok >> i) & 1u) ? ra[i] : 0.0;
+      double v = ((am >> i) & 1u) ? ra[i] : 0.0;
       if constexpr (KS) v *= rs[AKC ? 0 : i];
       if constexpr (AKC) As[buf][a_kk0][t / BK + (NT / BK) * i] = v;
       else As[buf][a_kk0 + (NT / BM) * i][t % BM] = v;
     }
 #pragma unroll
     for (int i = 0; i < NBL; ++i) {
-      const double v = ((b_ok >> i) & 1u) ? rb[i] : 0.0;
+      const double v = ((bm >> i) & 1u) ? rb[i] : 0.0;
       if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = v;
       else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = v;
     }

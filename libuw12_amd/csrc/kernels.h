@@ -37,6 +37,8 @@ void combine_dd(int nW, int nV, const double* t0, const double* t1, const double
                 double c1, const double* sW, const double* sV, double* Tb, double* Z, hipStream_t s);
 void symmetrise_add(const double* F, double* out, int N, int accumulate, hipStream_t s);
 void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
+// out = (in diag(sc))^T: in is rows x cols (column-major), out is cols x rows.
+void transpose_scale(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
 void fill(double* p, int64_t n, double v, hipStream_t s);
 
 }  // namespace uw12g

@@ -189,6 +189,25 @@ __global__ void scale_cols_kernel(const double* __restrict__ in, int64_t rows, i
     out[e] = in[e] * sc[e / rows];
 }
 
+// out[c + cols r] = in[r + rows c] sc[c]: 64 x 64 tiles through LDS, so both the
+// reads (along r) and the writes (along c) are coalesced.
+__global__ __launch_bounds__(256) void transpose_scale_kernel(const double* __restrict__ in, int64_t rows,
+                                                              int64_t cols, const double* __restrict__ sc,
+                                                              double* __restrict__ out) {
+  __shared__ double tile[64][65];
+  const int64_t r0 = int64_t(blockIdx.x) * 64, c0 = int64_t(blockIdx.y) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4 threads
+  for (int j = ty; j < 64; j += 4) {
+    const int64_t r = r0 + tx, c = c0 + j;
+    tile[j][tx] = (r < rows && c < cols) ? in[r + rows * c] * sc[c] : 0.0;
+  }
+  __syncthreads();
+  for (int j = ty; j < 64; j += 4) {
+    const int64_t c = c0 + tx, r = r0 + j;
+    if (r < rows && c < cols) out[c + cols * r] = tile[tx][j];
+  }
+}
+
 __global__ void fill_kernel(double* p, int64_t n, double v) {
   for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x)
     p[e] = v;
@@ -269,6 +288,15 @@ void combine_dd(int nW, int nV, const double* t0, const double* t1, const double
 
 void symmetrise_add(const double* F, double* out, int N, int accumulate, hipStream_t s) {
   hipLaunchKernelGGL(symmetrise_add_kernel, dim3(grid_for(int64_t(N) * N)), dim3(256), 0, s, F, out, N, accumulate);
+  UW12G_HIP(hipGetLastError());
+}
+
+void transpose_scale(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s) {
+  if (rows <= 0 || cols <= 0) return;
+  const int64_t gr = (rows + 63) / 64, gc = (cols + 63) / 64;
+  if (gr > 0x7fffffff || gc > 65535) throw Error(-2, "transpose_scale: matrix too large");
+  hipLaunchKernelGGL(transpose_scale_kernel, dim3(unsigned(gr), unsigned(gc)), dim3(256), 0, s, in, rows, cols, sc,
+                     out);
   UW12G_HIP(hipGetLastError());
 }
 

@@ -80,3 +80,66 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
     return -1;
   }
 }
+
+extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta,
+                               const double* A, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B,
+                               uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn,
+                               long long Cb, const double* ks, uw12g_map2 Km, int reps, double* ms) {
+  try {
+    UW12G_HIP(hipSetDevice(device));
+    GemmDesc d;
+    d.M = M;
+    d.N = N;
+    d.K = K;
+    d.batch = batch;
+    d.alpha = alpha;
+    d.beta = beta;
+    d.A = A;
+    d.Am = to_map(Am);
+    d.Ak = to_map(Ak);
+    d.Ab = Ab;
+    d.B = B;
+    d.Bk = to_map(Bk);
+    d.Bn = to_map(Bn);
+    d.Bb = Bb;
+    d.C = C;
+    d.Cm = to_map(Cm);
+    d.Cn = to_map(Cn);
+    d.Cb = Cb;
+    d.ks = ks;
+    d.Km = to_map(Km);
+    struct Stream {
+      hipStream_t s = nullptr;
+      ~Stream() {
+        if (s) {
+          (void)hipStreamSynchronize(s);
+          (void)hipStreamDestroy(s);
+        }
+      }
+    } guard;
+    UW12G_HIP(hipStreamCreateWithFlags(&guard.s, hipStreamNonBlocking));
+    Workspace ws;  // destroyed before the stream guard: the guard is declared first
+    hipStream_t s = guard.s;
+    gemm_f64(d, s, &ws);  // warm-up (also sizes the split-K workspace)
+    hipEvent_t e0, e1;
+    UW12G_HIP(hipEventCreate(&e0));
+    UW12G_HIP(hipEventCreate(&e1));
+    UW12G_HIP(hipEventRecord(e0, s));
+    for (int r = 0; r < reps; ++r) gemm_f64(d, s, &ws);
+    UW12G_HIP(hipEventRecord(e1, s));
+    UW12G_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    UW12G_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = double(t) / (reps > 0 ? reps : 1);
+    UW12G_HIP(hipEventDestroy(e0));
+    UW12G_HIP(hipEventDestroy(e1));
+    UW12G_HIP(hipStreamSynchronize(s));
+    return 0;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return -1;
+  }
+}

@@ -972,14 +972,18 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         const size_t cap = size_t(1) << 28;  // doubles (2 GiB) for the K~ block
         const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
         double* Kt = ctx->buf("Kt", size_t(MNo) * nj);
+        // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
+        // L contraction below reads its B operand along unit stride
+        double* X2sT = ctx->buf("X2sT", size_t(ao) * std::max(DW, DV));
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
           const uw12g_base_integrals* Y = pass == 0 ? V : W;
           const double* phitY = pass == 0 ? q.phitV : phitW;
           const double* phiY = pass == 0 ? q.phiV : q.phiW;
-          const double* X2s = pass == 0 ? W2s : V2s;
           const int DX = X->D, DY = Y->D;
+          timed(ctx, "3e_ind_fock_x2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DX,
+                [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
           for (int j0 = 0; j0 < a; j0 += nj) {
             const int jn = std::min(nj, a - j0);
             GemmDesc g;  // K~[mu', (beta, k) | j] = sum_B PhiT_Y[mu', j, B] sY_B Phi_Y[beta, k, B]
@@ -1001,7 +1005,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             g.Cn = lin(M);
             g.Cb = MNo;
             run_gemm(ctx, g, "3e_ind_fock_kt");
-            GemmDesc h;  // L[(mu', beta), A] += sum_{(k, j)} K~[(mu', beta), k | j] X2s[j + a k, A]
+            GemmDesc h;  // L[(mu', beta), A] += sum_{(k, j)} K~[(mu', beta), k | j] X2sT[A, j + a k]
             h.M = int(MN);
             h.N = DX;
             h.K = o * jn;
@@ -1009,9 +1013,9 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             h.A = Kt;
             h.Am = lin(1);
             h.Ak = lin(int64_t(MN));
-            h.B = X2s + j0;
-            h.Bk = two(uint32_t(o), a, 1);
-            h.Bn = lin(ao);
+            h.B = X2sT + int64_t(DX) * j0;
+            h.Bk = two(uint32_t(o), int64_t(DX) * a, DX);
+            h.Bn = lin(1);
             h.C = L;
             h.Cm = lin(1);
             h.Cn = lin(int64_t(MN));
@@ -1036,6 +1040,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         }
         ctx->release("L");
         ctx->release("Kt");
+        ctx->release("X2sT");
         // fock_k (indirect_utils.cpp:199-259), blocked over j
         const int64_t MaN = int64_t(M) * a * N;
         const int nj2 = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MaN, 1))));
