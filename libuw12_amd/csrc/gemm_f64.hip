@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 
 #include "uw12g_internal.h"
 
@@ -29,26 +30,48 @@ struct Params {
   int tiles_m, tiles_n, splits, kchunk;
   double* out;  // split-K partials (dense, M x N per split and batch) or nullptr
   int remap;    // 1: spread consecutive tiles over XCDs in contiguous chunks
+  int fast;     // 1: buffer-load main path (fast_ok)
+  int nfast;    // 1: tiles ordered with the column index fastest
 };
 
-// One 16-deep K slab per iteration, double-buffered in LDS (one barrier per
-// slab).  Global loads for slab t+1 are issued before the MFMAs of slab t and
-// consumed only when written to LDS after them, so their latency overlaps the
-// matrix work: every load address is clamped to an in-bounds element (offset 0)
-// when out of range and the predicate is applied at the LDS write.
-#ifndef UW12G_LDS_PAD
-#define UW12G_LDS_PAD 4
-#endif
+// Buffer-resource loads.  The resource base is a wave-uniform address (scalar
+// registers); the per-lane part of an element address is a 32-bit byte offset
+// fixed for the whole K loop and the per-element part a scalar byte offset, so
+// a load costs no vector address arithmetic.  A resource built with
+// num_records = 0 reads zero, which is how k beyond the end is masked.
+__device__ inline __amdgpu_buffer_rsrc_t rsrc(const double* base, bool valid = true) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, valid ? 0x7fffffff : 0, 0x00020000);
+}
+__device__ inline double ld_b(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int32_t uniform_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uniform_off, 0));
+}
+constexpr uint32_t kOutOfRange = 0x80000000u;  // lane offset past num_records: the load reads zero
+
+// K runs in 16-deep slabs, double-buffered in LDS with one barrier per slab.
+// Each slab is split into 4 load groups, one per 4-deep MFMA step: during step
+// q of slab t a wave writes group q of slab t+1 (loaded one slab earlier) into
+// the other LDS buffer and issues the group-q loads of slab t+2, so global
+// loads, LDS writes and MFMAs interleave.  On CDNA4 the FP64 MFMA shares issue
+// with vector ALU work, so the main path (p.fast: affine k maps, 32-bit
+// offsets) loads through buffer resources with no predicates: rows and columns
+// past the matrix read row / column 0 (they only feed outputs that are not
+// stored) and k past the end reads zero.  Other descriptors take a general
+// path with 64-bit offsets and clamped, zeroed loads.
 template <int BM, int BN, bool AKC, bool BKC, bool KS>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
-  constexpr int PAD = UW12G_LDS_PAD;
+  constexpr int PAD = 4;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int NA = BM * BK / NT;  // A elements loaded per thread and slab
   constexpr int NBL = BN * BK / NT;
+  constexpr int G = 4;              // load groups per slab
+  constexpr int GA = NA / G, GB = NBL / G;
   __shared__ double As[2][BK][BM + PAD];
   __shared__ double Bs[2][BK][BN + PAD];
-
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -57,7 +80,10 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     const int per = (p.tiles_m * p.tiles_n) >> 3;
     tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
   }
-  const int tm = tile % p.tiles_m, tn = tile / p.tiles_m;
+  // column-fastest order for tall outputs: the tiles that share an A panel run
+  // together (on one XCD when remapped), so the panel is read from HBM once
+  const int tm = p.nfast ? tile / p.tiles_n : tile % p.tiles_m;
+  const int tn = p.nfast ? tile % p.tiles_n : tile / p.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
   const int b = blockIdx.z;
   const int k_begin = blockIdx.y * p.kchunk;
@@ -69,96 +95,30 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
 
   // ---- per-thread load assignment (fixed over the K loop)
   // A: AKC -> k = t % BK, rows t/BK + (NT/BK) i ; else row t % BM, k = t/BM + (NT/BM) i.
-  // In the second form every k a wave loads is wave-uniform (BM, BN >= 64), so
-  // its offset, range test and scale factor are computed once per wave on the
-  // scalar unit and the vector work per element is one address add.
+  // In the second form every k a wave loads is wave-uniform (BM, BN >= 64).
+  // Rows past M use row 0's offset.
   constexpr int NAM = AKC ? NA : 1;
-  int64_t a_moff[NAM];
-  uint32_t a_mok = 0;
-  const int a_kk0 = AKC ? t % BK : __builtin_amdgcn_readfirstlane(t / BM);
-#pragma unroll
-  for (int i = 0; i < NAM; ++i) {
-    const int m = m0 + (AKC ? t / BK + (NT / BK) * i : t % BM);
-    const bool ok = m < d.M;
-    a_mok |= uint32_t(ok) << i;
-    a_moff[i] = ok ? d.Am.off(m) : 0;
-  }
   constexpr int NBN = BKC ? NBL : 1;
-  int64_t b_noff[NBN];
-  uint32_t b_nok = 0;
+  const int a_kk0 = AKC ? t % BK : __builtin_amdgcn_readfirstlane(t / BM);
   const int b_kk0 = BKC ? t % BK : __builtin_amdgcn_readfirstlane(t / BN);
-#pragma unroll
-  for (int i = 0; i < NBN; ++i) {
-    const int n = n0 + (BKC ? t / BK + (NT / BK) * i : t % BN);
-    const bool ok = n < d.N;
-    b_nok |= uint32_t(ok) << i;
-    b_noff[i] = ok ? d.Bn.off(n) : 0;
-  }
+  auto a_row = [&](int i) { return m0 + (AKC ? t / BK + (NT / BK) * i : t % BM); };
+  auto b_col = [&](int i) { return n0 + (BKC ? t / BK + (NT / BK) * i : t % BN); };
 
-  double ra[NA], rb[NBL], rs[AKC ? 1 : NA];
-  uint32_t a_ok = 0, b_ok = 0;
-  auto load = [&](int k0) {
-    a_ok = 0;
-    b_ok = 0;
-    if constexpr (AKC) {
-      const int k = k0 + a_kk0;
-      const bool kok = k < k_end;
-      const int64_t ko = kok ? d.Ak.off(k) : 0;
-      if constexpr (KS) rs[0] = Kp[kok ? d.Km.off(k) : 0];
+  double ra[NA], rb[NBL], rs[AKC ? G : NA];
+
+  auto store = [&](auto qc, int buf) {
+    constexpr int q = decltype(qc)::value;
 #pragma unroll
-      for (int i = 0; i < NA; ++i) {
-        const bool ok = kok && ((a_mok >> i) & 1u);
-        a_ok |= uint32_t(ok) << i;
-        ra[i] = Ap[ok ? a_moff[i] + ko : 0];
-      }
-    } else {
-      // row m = 0 and column k = 0 exist, so a clamped offset stays in bounds
-#pragma unroll
-      for (int i = 0; i < NA; ++i) {
-        const int k = k0 + a_kk0 + (NT / BM) * i;  // wave-uniform
-        const bool kok = k < k_end;
-        a_ok |= uint32_t(kok) << i;
-        ra[i] = Ap[a_moff[0] + (kok ? d.Ak.off(k) : 0)];
-        if constexpr (KS) rs[i] = Kp[kok ? d.Km.off(k) : 0];
-      }
-    }
-    if constexpr (BKC) {
-      const int k = k0 + b_kk0;
-      const bool kok = k < k_end;
-      const int64_t ko = kok ? d.Bk.off(k) : 0;
-#pragma unroll
-      for (int i = 0; i < NBL; ++i) {
-        const bool ok = kok && ((b_nok >> i) & 1u);
-        b_ok |= uint32_t(ok) << i;
-        rb[i] = Bp[ok ? b_noff[i] + ko : 0];
-      }
-    } else {
-#pragma unroll
-      for (int i = 0; i < NBL; ++i) {
-        const int k = k0 + b_kk0 + (NT / BN) * i;  // wave-uniform
-        const bool kok = k < k_end;
-        b_ok |= uint32_t(kok) << i;
-        rb[i] = Bp[b_noff[0] + (kok ? d.Bk.off(k) : 0)];
-      }
-    }
-  };
-  // non-k-contiguous forms: a_ok / b_ok hold the uniform k tests, a_mok / b_nok the lane's row test
-  const uint32_t a_lane = AKC ? ~0u : (a_mok & 1u ? ~0u : 0u);
-  const uint32_t b_lane = BKC ? ~0u : (b_nok & 1u ? ~0u : 0u);
-  auto store = [&](int buf) {
-    const uint32_t am = a_ok & a_lane, bm = b_ok & b_lane;
-#pragma unroll
-    for (int i = 0; i < NA; ++i) {
-      double v = ((am >> i) & 1u) ? ra[i] : 0.0;
-      if constexpr (KS) v *= rs[AKC ? 0 : i];
+    for (int i = q * GA; i < (q + 1) * GA; ++i) {
+      double v = ra[i];
+      if constexpr (KS) v *= rs[AKC ? q : i];
       if constexpr (AKC) As[buf][a_kk0][t / BK + (NT / BK) * i] = v;
       else As[buf][a_kk0 + (NT / BM) * i][t % BM] = v;
     }
 #pragma unroll
-    for (int i = 0; i < NBL; ++i) {
-      const double v = ((bm >> i) & 1u) ? rb[i] : 0.0;
-      if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = v;
-      else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = v;
+    for (int i = q * GB; i < (q + 1) * GB; ++i) {
+      if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = rb[i];
+      else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = rb[i];
     }
   };
 
@@ -171,27 +131,171 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
 
-  if (k_begin < k_end) {
-    load(k_begin);
-    int buf = 0;
-    for (int k0 = k_begin; k0 < k_end; k0 += BK) {
-      store(buf);
+  auto kstep = [&](auto qc, int buf) {
+    constexpr int q = decltype(qc)::value;
+    double af[MB], bf[NB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) af[i] = As[buf][4 * q + lk][wm0 + 16 * i + lr];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) bf[j] = Bs[buf][4 * q + lk][wn0 + 16 * j + lr];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+  };
+  // Slab pipeline over [k_begin, k_end) with a group loader ld(q, k0).
+  auto pipeline = [&](auto&& ld) {
+    ld(I0{}, k_begin);
+    ld(I1{}, k_begin);
+    ld(I2{}, k_begin);
+    ld(I3{}, k_begin);
+    store(I0{}, 0);
+    store(I1{}, 0);
+    store(I2{}, 0);
+    store(I3{}, 0);
+    __syncthreads();
+    ld(I0{}, k_begin + BK);
+    ld(I1{}, k_begin + BK);
+    ld(I2{}, k_begin + BK);
+    ld(I3{}, k_begin + BK);
+    const int nslab = (k_end - k_begin + BK - 1) / BK;
+    for (int s = 0; s < nslab; ++s) {
+      // MFMAs from buffer cur; the other buffer receives slab s+1 (loaded one
+      // slab earlier) and the registers slab s+2
+      const int cur = s & 1, nx = cur ^ 1;
+      const int k2 = k_begin + (s + 2) * BK;
+      kstep(I0{}, cur);
+      store(I0{}, nx);
+      ld(I0{}, k2);
+      kstep(I1{}, cur);
+      store(I1{}, nx);
+      ld(I1{}, k2);
+      kstep(I2{}, cur);
+      store(I2{}, nx);
+      ld(I2{}, k2);
+      kstep(I3{}, cur);
+      store(I3{}, nx);
+      ld(I3{}, k2);
       __syncthreads();
-      if (k0 + BK < k_end) load(k0 + BK);
+    }
+  };
+
+  if (k_begin < k_end) {
+    if (p.fast) {
+      uint32_t a_lo[NAM], b_lo[NBN];  // lane byte offsets from the slab's base
 #pragma unroll
-      for (int ks = 0; ks < BK; ks += 4) {
-        double af[MB], bf[NB];
-#pragma unroll
-        for (int i = 0; i < MB; ++i) af[i] = As[buf][ks + lk][wm0 + 16 * i + lr];
-#pragma unroll
-        for (int j = 0; j < NB; ++j) bf[j] = Bs[buf][ks + lk][wn0 + 16 * j + lr];
-#pragma unroll
-        for (int i = 0; i < MB; ++i)
-#pragma unroll
-          for (int j = 0; j < NB; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+      for (int i = 0; i < NAM; ++i) {
+        const int m = a_row(i);
+        a_lo[i] = uint32_t(8 * ((m < d.M ? d.Am.off(m) : 0) + (AKC ? int64_t(a_kk0) * d.Ak.s0 : 0)));
       }
-      buf ^= 1;
+#pragma unroll
+      for (int i = 0; i < NBN; ++i) {
+        const int n = b_col(i);
+        b_lo[i] = uint32_t(8 * ((n < d.N ? d.Bn.off(n) : 0) + (BKC ? int64_t(b_kk0) * d.Bk.s0 : 0)));
+      }
+      const uint32_t k_lo = uint32_t(8 * (AKC ? int64_t(a_kk0) * d.Km.s0 : 0));
+      const int32_t a_ks = int32_t(8 * (NT / BM) * d.Ak.s0), b_ks = int32_t(8 * (NT / BN) * d.Bk.s0);
+      const int64_t k_kstep = int64_t(NT / BM) * d.Km.s0;
+      // slab k0 is affine (off(k0 + dk) = off(k0) + dk s0), so one offset per slab
+      pipeline([&](auto qc, int k0) {
+        constexpr int q = decltype(qc)::value;
+        const bool full = k0 + BK <= k_end;  // wave-uniform
+        if constexpr (AKC) {
+          const auto r = rsrc(Ap + d.Ak.off(k0));
+          const bool kok = k0 + a_kk0 < k_end;
+          if (full) {
+            if constexpr (KS) rs[q] = ld_b(rsrc(Kp + d.Km.off(k0)), k_lo, 0);
+#pragma unroll
+            for (int i = q * GA; i < (q + 1) * GA; ++i) ra[i] = ld_b(r, a_lo[i], 0);
+          } else {
+            if constexpr (KS) rs[q] = ld_b(rsrc(Kp + d.Km.off(k0)), kok ? k_lo : kOutOfRange, 0);
+#pragma unroll
+            for (int i = q * GA; i < (q + 1) * GA; ++i) ra[i] = ld_b(r, kok ? a_lo[i] : kOutOfRange, 0);
+          }
+        } else {
+          const int kb = k0 + a_kk0;
+#pragma unroll
+          for (int i = q * GA; i < (q + 1) * GA; ++i) {
+            const bool kok = kb + (NT / BM) * i < k_end;
+            ra[i] = ld_b(rsrc(Ap + d.Ak.off(kb), kok), a_lo[0], i * a_ks);
+            if constexpr (KS) rs[i] = Kp[kok ? d.Km.off(kb) + i * k_kstep : 0];
+          }
+        }
+        if constexpr (BKC) {
+          const auto r = rsrc(Bp + d.Bk.off(k0));
+          const bool kok = k0 + b_kk0 < k_end;
+          if (full) {
+#pragma unroll
+            for (int i = q * GB; i < (q + 1) * GB; ++i) rb[i] = ld_b(r, b_lo[i], 0);
+          } else {
+#pragma unroll
+            for (int i = q * GB; i < (q + 1) * GB; ++i) rb[i] = ld_b(r, kok ? b_lo[i] : kOutOfRange, 0);
+          }
+        } else {
+          const int kb = k0 + b_kk0;
+#pragma unroll
+          for (int i = q * GB; i < (q + 1) * GB; ++i) {
+            const bool kok = kb + (NT / BN) * i < k_end;
+            rb[i] = ld_b(rsrc(Bp + d.Bk.off(kb), kok), b_lo[0], i * b_ks);
+          }
+        }
+      });
+    } else {
+      // general path: 64-bit offsets, clamped in-bounds addresses (row / column
+      // 0, k = 0) and zeroed values for k past the end
+      int64_t a_moff[NAM], b_noff[NBN];
+#pragma unroll
+      for (int i = 0; i < NAM; ++i) {
+        const int m = a_row(i);
+        a_moff[i] = m < d.M ? d.Am.off(m) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < NBN; ++i) {
+        const int n = b_col(i);
+        b_noff[i] = n < d.N ? d.Bn.off(n) : 0;
+      }
+      pipeline([&](auto qc, int k0) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (AKC) {
+          const int k = k0 + a_kk0;
+          const bool kok = k < k_end;
+          const int64_t ko = kok ? d.Ak.off(k) : 0;
+          if constexpr (KS) rs[q] = Kp[kok ? d.Km.off(k) : 0];
+#pragma unroll
+          for (int i = q * GA; i < (q + 1) * GA; ++i) {
+            const double v = Ap[a_moff[i] + ko];
+            ra[i] = kok ? v : 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int i = q * GA; i < (q + 1) * GA; ++i) {
+            const int k = k0 + a_kk0 + (NT / BM) * i;  // wave-uniform
+            const bool kok = k < k_end;
+            const double v = Ap[a_moff[0] + (kok ? d.Ak.off(k) : 0)];
+            ra[i] = kok ? v : 0.0;
+            if constexpr (KS) rs[i] = Kp[kok ? d.Km.off(k) : 0];
+          }
+        }
+        if constexpr (BKC) {
+          const int k = k0 + b_kk0;
+          const bool kok = k < k_end;
+          const int64_t ko = kok ? d.Bk.off(k) : 0;
+#pragma unroll
+          for (int i = q * GB; i < (q + 1) * GB; ++i) {
+            const double v = Bp[b_noff[i] + ko];
+            rb[i] = kok ? v : 0.0;
+          }
+        } else {
+#pragma unroll
+          for (int i = q * GB; i < (q + 1) * GB; ++i) {
+            const int k = k0 + b_kk0 + (NT / BN) * i;  // wave-uniform
+            const bool kok = k < k_end;
+            const double v = Bp[b_noff[0] + (kok ? d.Bk.off(k) : 0)];
+            rb[i] = kok ? v : 0.0;
+          }
+        }
+      });
     }
   }
 
@@ -286,6 +390,28 @@ void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
 // Unit stride on the low level of a map (contiguous runs along that index).
 bool unit(const Map2& m) { return m.s0 == 1 && m.n0 >= 2; }
 
+// A k map is affine inside every slab when its inner period is a multiple of
+// BK (slabs start at multiples of BK) or it has a single level.
+bool slab_affine(const Map2& m) { return m.n0 == 0xffffffffu || m.n0 % BK == 0; }
+bool nonneg(const Map2& m) { return m.s0 >= 0 && m.s1 >= 0; }
+}  // namespace
+int64_t map_max(const Map2& m, int64_t extent);
+namespace {
+
+// Conditions for the unpredicated interior path of gemm_f64_kernel: affine k
+// maps and every lane byte offset (row or column offset, plus the lane's k
+// offset inside a slab in the k-contiguous forms) below 2^32.
+bool fast_ok(const GemmDesc& d, bool akc, bool bkc) {
+  if (!slab_affine(d.Ak) || !slab_affine(d.Bk) || (d.ks && !slab_affine(d.Km))) return false;
+  if (!nonneg(d.Am) || !nonneg(d.Ak) || !nonneg(d.Bk) || !nonneg(d.Bn) || (d.ks && !nonneg(d.Km))) return false;
+  // lane plus uniform byte offset of every buffer load below num_records = 2^31 - 1
+  const int64_t lim = (int64_t(1) << 28) - 1;  // doubles
+  const int64_t a = map_max(d.Am, d.M) + (BK - 1) * d.Ak.s0;
+  const int64_t b = map_max(d.Bn, d.N) + (BK - 1) * d.Bk.s0;
+  const int64_t k = (d.ks && akc) ? (BK - 1) * d.Km.s0 : 0;
+  return a < lim && b < lim && k < lim;
+}
+
 }  // namespace
 
 int64_t map_max(const Map2& m, int64_t extent) {
@@ -343,6 +469,8 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   p.out = nullptr;
   if (splits > 1) p.out = ws->get(size_t(splits) * d.batch * size_t(d.M) * d.N, stream);
   p.remap = (tiles % 8 == 0 && tiles >= 64) ? 1 : 0;
+  p.fast = fast_ok(d, akc, bkc) ? 1 : 0;
+  p.nfast = (p.tiles_n <= 16 && p.tiles_m >= 4 * p.tiles_n) ? 1 : 0;
   dim3 grid(unsigned(tiles), unsigned(splits), unsigned(d.batch));
   const bool timing = prof && prof->on;
   ProfileRec rec;

@@ -159,6 +159,9 @@ using namespace uw12g;
 
 struct uw12g_context {
   int device = 0;
+  // doubles allowed for one blocked intermediate of the indirect terms (K~,
+  // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
+  size_t block_cap = size_t(1) << 28;
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
   Workspace gemm_ws;
@@ -917,7 +920,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k
       {
         const int64_t Ma = int64_t(M) * a;
-        const size_t cap = size_t(1) << 27;  // doubles per block buffer (1 GiB)
+        const size_t cap = ctx->block_cap;
         int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
         double* GWb = ctx->buf("e3_gw", size_t(Ma) * a * nk);
         double* GVb = ctx->buf("e3_gv", size_t(Ma) * a * nk);
@@ -969,7 +972,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         run_gemm(ctx, p, "3e_abs_apply");
         double* L = ctx->buf("L", MN * std::max(DW, DV));
         const int64_t MNo = int64_t(MN) * o;
-        const size_t cap = size_t(1) << 28;  // doubles (2 GiB) for the K~ block
+        const size_t cap = ctx->block_cap;
         const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
         double* Kt = ctx->buf("Kt", size_t(MNo) * nj);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
@@ -1144,6 +1147,9 @@ int uw12g_context_create(int device, uw12g_context** out) {
     c->device = device;
     UW12G_HIP(hipSetDevice(device));
     UW12G_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    size_t free_b = 0, total_b = 0;
+    UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
+    c->block_cap = std::max<size_t>(size_t(1) << 24, std::min<size_t>(total_b / 16, size_t(16) << 30) / 8);
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Error(-3, "rocblas_create_handle failed");
     rocblas_set_stream(c->rb, c->stream);
     *out = c.release();
