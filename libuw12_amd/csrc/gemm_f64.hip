@@ -30,22 +30,40 @@ struct Params {
   int tiles_m, tiles_n, splits, kchunk;
   double* out;  // split-K partials (dense, M x N per split and batch) or nullptr
   int remap;    // 1: spread consecutive tiles over XCDs in contiguous chunks
-  int fast;     // 1: buffer-load main path (fast_ok)
+  int fast;     // 1: buffer-load main path (fast_ok); every slab of every chunk is full
   int nfast;    // 1: tiles ordered with the column index fastest
+  int kbase;    // first k of chunk 0
 };
 
 // Buffer-resource loads.  The resource base is a wave-uniform address (scalar
 // registers); the per-lane part of an element address is a 32-bit byte offset
 // fixed for the whole K loop and the per-element part a scalar byte offset, so
-// a load costs no vector address arithmetic.  A resource built with
-// num_records = 0 reads zero, which is how k beyond the end is masked.
+// a load costs no vector address arithmetic.
+// A resource with num_records = 0 reads zero without touching memory.
 __device__ inline __amdgpu_buffer_rsrc_t rsrc(const double* base, bool valid = true) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(base), (short)0, valid ? 0x7fffffff : 0, 0x00020000);
 }
 __device__ inline double ld_b(__amdgpu_buffer_rsrc_t r, uint32_t lane_off, int32_t uniform_off) {
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uniform_off, 0));
 }
-constexpr uint32_t kOutOfRange = 0x80000000u;  // lane offset past num_records: the load reads zero
+
+// Instruction order inside one slab of the main path, as scheduling groups:
+// the first step's LDS reads, then every MFMA followed by its share of the
+// slab's LDS writes, global loads, LDS reads and scalar work, spread evenly,
+// so that each wave issues memory work in the shadow of its own MFMAs.
+template <int NM, int NW, int NG, int NR0, int NR, int NS, int NV>
+__device__ inline void slab_schedule() {
+  __builtin_amdgcn_sched_group_barrier(0x100, NR0, 0);  // LDS reads of the first step
+#pragma unroll
+  for (int j = 0; j < NM; ++j) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+    if ((j * NW) / NM != ((j + 1) * NW) / NM) __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);  // LDS write
+    if ((j * NG) / NM != ((j + 1) * NG) / NM) __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);  // global load
+    if ((j * NR) / NM != ((j + 1) * NR) / NM) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // LDS read
+    if ((j * NV) / NM != ((j + 1) * NV) / NM) __builtin_amdgcn_sched_group_barrier(0x002, 1, 0);  // VALU
+    if ((j * NS) / NM != ((j + 1) * NS) / NM) __builtin_amdgcn_sched_group_barrier(0x004, 1, 0);  // SALU
+  }
+}
 
 // K runs in 16-deep slabs, double-buffered in LDS with one barrier per slab.
 // Each slab is split into 4 load groups, one per 4-deep MFMA step: during step
@@ -53,10 +71,13 @@ constexpr uint32_t kOutOfRange = 0x80000000u;  // lane offset past num_records: 
 // the other LDS buffer and issues the group-q loads of slab t+2, so global
 // loads, LDS writes and MFMAs interleave.  On CDNA4 the FP64 MFMA shares issue
 // with vector ALU work, so the main path (p.fast: affine k maps, 32-bit
-// offsets) loads through buffer resources with no predicates: rows and columns
-// past the matrix read row / column 0 (they only feed outputs that are not
-// stored) and k past the end reads zero.  Other descriptors take a general
-// path with 64-bit offsets and clamped, zeroed loads.
+// offsets) loads through buffer resources with no vector predicates: rows and
+// columns past the matrix read row / column 0 (they only feed outputs that are
+// not stored); k past the end reads zero through an empty resource where k is
+// wave-uniform, and where k runs along the lanes every slab is whole (the host
+// runs the k remainder separately) and the look-ahead past the last slab
+// re-reads it.  Other descriptors take a general path with 64-bit offsets and
+// clamped, zeroed loads.
 template <int BM, int BN, bool AKC, bool BKC, bool KS>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   constexpr int PAD = 4;
@@ -86,7 +107,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const int tn = p.nfast ? tile % p.tiles_n : tile / p.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
   const int b = blockIdx.z;
-  const int k_begin = blockIdx.y * p.kchunk;
+  const int k_begin = p.kbase + blockIdx.y * p.kchunk;
   const int k_end = min(d.K, k_begin + p.kchunk);
 
   const double* __restrict__ Ap = d.A + d.Ab * b;
@@ -144,8 +165,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       for (int j = 0; j < NB; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
   };
-  // Slab pipeline over [k_begin, k_end) with a group loader ld(q, k0).
-  auto pipeline = [&](auto&& ld) {
+  // Slab pipeline over [k_begin, k_end) with a group loader ld(q, k0); k0 of
+  // a look-ahead slab may lie past the end (the loader clamps or masks it).
+  auto pipeline = [&](auto fastc, auto&& ld) {
     ld(I0{}, k_begin);
     ld(I1{}, k_begin);
     ld(I2{}, k_begin);
@@ -177,6 +199,13 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       kstep(I3{}, cur);
       store(I3{}, nx);
       ld(I3{}, k2);
+      if constexpr (decltype(fastc)::value) {
+        // per slab: 4 MB NB MFMAs, NA + NBL writes and loads, 2 (MB + NB) LDS
+        // reads (as read2), about 4 scalar ops per MFMA
+        slab_schedule<4 * MB * NB, NA + NBL, NA + NBL + (KS ? (AKC ? G : NA) : 0), (MB + NB) / 2,
+                      3 * (MB + NB) / 2, 4 * MB * NB, KS ? NA : 0>();
+      }
+      __builtin_amdgcn_sched_barrier(0);
       __syncthreads();
     }
   };
@@ -198,40 +227,29 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       const int32_t a_ks = int32_t(8 * (NT / BM) * d.Ak.s0), b_ks = int32_t(8 * (NT / BN) * d.Bk.s0);
       const int64_t k_kstep = int64_t(NT / BM) * d.Km.s0;
       // slab k0 is affine (off(k0 + dk) = off(k0) + dk s0), so one offset per slab
-      pipeline([&](auto qc, int k0) {
+      const int k_last = k_end - BK;
+      pipeline(std::true_type{}, [&](auto qc, int k0) {
         constexpr int q = decltype(qc)::value;
-        const bool full = k0 + BK <= k_end;  // wave-uniform
         if constexpr (AKC) {
-          const auto r = rsrc(Ap + d.Ak.off(k0));
-          const bool kok = k0 + a_kk0 < k_end;
-          if (full) {
-            if constexpr (KS) rs[q] = ld_b(rsrc(Kp + d.Km.off(k0)), k_lo, 0);
+          const int kc = min(k0, k_last);  // look-ahead past the end re-reads the last slab
+          const auto r = rsrc(Ap + d.Ak.off(kc));
+          if constexpr (KS) rs[q] = ld_b(rsrc(Kp + d.Km.off(kc)), k_lo, 0);
 #pragma unroll
-            for (int i = q * GA; i < (q + 1) * GA; ++i) ra[i] = ld_b(r, a_lo[i], 0);
-          } else {
-            if constexpr (KS) rs[q] = ld_b(rsrc(Kp + d.Km.off(k0)), kok ? k_lo : kOutOfRange, 0);
-#pragma unroll
-            for (int i = q * GA; i < (q + 1) * GA; ++i) ra[i] = ld_b(r, kok ? a_lo[i] : kOutOfRange, 0);
-          }
+          for (int i = q * GA; i < (q + 1) * GA; ++i) ra[i] = ld_b(r, a_lo[i], 0);
         } else {
-          const int kb = k0 + a_kk0;
+          const int kb = k0 + a_kk0;  // wave-uniform
+          const int64_t kbo = KS ? d.Km.off(kb) : 0;
 #pragma unroll
           for (int i = q * GA; i < (q + 1) * GA; ++i) {
             const bool kok = kb + (NT / BM) * i < k_end;
             ra[i] = ld_b(rsrc(Ap + d.Ak.off(kb), kok), a_lo[0], i * a_ks);
-            if constexpr (KS) rs[i] = Kp[kok ? d.Km.off(kb) + i * k_kstep : 0];
+            if constexpr (KS) rs[i] = Kp[kok ? kbo + i * k_kstep : 0];
           }
         }
         if constexpr (BKC) {
-          const auto r = rsrc(Bp + d.Bk.off(k0));
-          const bool kok = k0 + b_kk0 < k_end;
-          if (full) {
+          const auto r = rsrc(Bp + d.Bk.off(min(k0, k_last)));
 #pragma unroll
-            for (int i = q * GB; i < (q + 1) * GB; ++i) rb[i] = ld_b(r, b_lo[i], 0);
-          } else {
-#pragma unroll
-            for (int i = q * GB; i < (q + 1) * GB; ++i) rb[i] = ld_b(r, kok ? b_lo[i] : kOutOfRange, 0);
-          }
+          for (int i = q * GB; i < (q + 1) * GB; ++i) rb[i] = ld_b(r, b_lo[i], 0);
         } else {
           const int kb = k0 + b_kk0;
 #pragma unroll
@@ -255,7 +273,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
         const int n = b_col(i);
         b_noff[i] = n < d.N ? d.Bn.off(n) : 0;
       }
-      pipeline([&](auto qc, int k0) {
+      pipeline(std::false_type{}, [&](auto qc, int k0) {
         constexpr int q = decltype(qc)::value;
         if constexpr (AKC) {
           const int k = k0 + a_kk0;
@@ -432,44 +450,50 @@ static void check_desc(const GemmDesc& d) {
   check_extent(d.C, d.Cb * (d.batch - 1) + map_max(d.Cm, d.M) + map_max(d.Cn, d.N), "gemm C");
 }
 
-void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
-  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  check_desc(d);
-  if (d.batch > 65535) throw Error(-2, "gemm batch exceeds 65535");
-  if (d.K <= 0) {  // C <- beta C
-    GemmDesc z = d;
-    z.K = 0;
-    gemm_f64_naive(z, stream);
-    return;
-  }
-  // Tile shape: large tiles for large outputs, small for thin ones.
-  int BM = d.M >= 96 ? 128 : 64;
-  int BN = d.N >= 96 ? 128 : 64;
-  // coalesce along k only when that index has the unit stride and the other does not
-  const bool akc = unit(d.Ak) && !unit(d.Am);
-  const bool bkc = unit(d.Bk) && !unit(d.Bn);
+namespace {
+
+// One launch over k in [kbase, d.K): split over K when the grid would leave
+// most of the 256 CUs idle (kbase == 0 only).
+void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hipStream_t stream, Workspace* ws,
+                 Profiler* prof) {
+  // Tile shape: the best product of padding efficiency (useful fraction of
+  // the padded output) and the relative speed of the tile shape.
+  int BM = 128, BN = 128;
+  double best = -1.0;
+  for (int bm : {128, 64})
+    for (int bn : {128, 64}) {
+      const double padded = double((d.M + bm - 1) / bm * bm) * double((d.N + bn - 1) / bn * bn);
+      const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? 0.7 : 0.9;
+      const double score = speed * double(d.M) * double(d.N) / padded;
+      if (score > best + 1e-12) {
+        best = score;
+        BM = bm;
+        BN = bn;
+      }
+    }
   Params p;
   p.d = d;
   p.tiles_m = (d.M + BM - 1) / BM;
   p.tiles_n = (d.N + BN - 1) / BN;
   const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
   if (tiles > 0x7fffffff) throw Error(-2, "gemm grid too large");
-  // Split K when the grid would leave most of the 256 CUs idle.
+  const int klen = d.K - kbase;
   int splits = 1;
   const int64_t wgs = tiles * d.batch;
-  const int kt = (d.K + BK - 1) / BK;
-  if (wgs < 512 && kt >= 16 && ws) {
+  const int kt = (klen + BK - 1) / BK;
+  if (kbase == 0 && wgs < 512 && kt >= 16 && ws) {
     splits = int(std::min<int64_t>((1024 + wgs - 1) / wgs, kt / 8));
     splits = std::max(1, std::min(splits, 64));
     while (splits > 1 && int64_t(splits) * d.batch > 65535) --splits;
   }
   p.kchunk = ((kt + splits - 1) / splits) * BK;
-  splits = (d.K + p.kchunk - 1) / p.kchunk;
+  splits = (klen + p.kchunk - 1) / p.kchunk;
   p.splits = splits;
+  p.kbase = kbase;
   p.out = nullptr;
   if (splits > 1) p.out = ws->get(size_t(splits) * d.batch * size_t(d.M) * d.N, stream);
   p.remap = (tiles % 8 == 0 && tiles >= 64) ? 1 : 0;
-  p.fast = fast_ok(d, akc, bkc) ? 1 : 0;
+  p.fast = fast ? 1 : 0;
   p.nfast = (p.tiles_n <= 16 && p.tiles_m >= 4 * p.tiles_n) ? 1 : 0;
   dim3 grid(unsigned(tiles), unsigned(splits), unsigned(d.batch));
   const bool timing = prof && prof->on;
@@ -490,7 +514,7 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
     if (timing) kname = cfg_name<64, 64>(akc, bkc, d.ks != nullptr);
   }
   UW12G_HIP(hipGetLastError());
-  if (timing) prof->end(rec, stream, kname, 2.0 * d.M * double(d.N) * d.K * d.batch, 0.0);
+  if (timing) prof->end(rec, stream, kname, 2.0 * d.M * double(d.N) * klen * d.batch, 0.0);
   if (splits > 1) {
     const int64_t MN = int64_t(d.M) * d.N;
     dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
@@ -499,6 +523,38 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
     hipLaunchKernelGGL(splitk_reduce_kernel, g, dim3(256), 0, stream, d, p.out, splits);
     UW12G_HIP(hipGetLastError());
     if (timing) prof->end(r2, stream, "splitk_reduce_kernel", double(splits) * d.batch * MN, 8.0 * (splits + 1) * d.batch * MN);
+  }
+}
+
+}  // namespace
+
+void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  check_desc(d);
+  if (d.batch > 65535) throw Error(-2, "gemm batch exceeds 65535");
+  if (d.K <= 0) {  // C <- beta C
+    GemmDesc z = d;
+    z.K = 0;
+    gemm_f64_naive(z, stream);
+    return;
+  }
+  // coalesce along k only when that index has the unit stride and the other does not
+  const bool akc = unit(d.Ak) && !unit(d.Am);
+  const bool bkc = unit(d.Bk) && !unit(d.Bn);
+  // with k along the lanes the unpredicated path needs whole slabs: the k
+  // remainder then runs on the general path, accumulated onto them
+  const int krem = (akc || bkc) ? d.K % BK : 0;
+  if (d.K >= BK && fast_ok(d, akc, bkc)) {
+    GemmDesc m = d;
+    m.K = d.K - krem;
+    launch_gemm(m, 0, true, akc, bkc, stream, ws, prof);
+    if (krem) {
+      GemmDesc r = d;
+      r.beta = 1.0;
+      launch_gemm(r, d.K - krem, false, akc, bkc, stream, ws, prof);
+    }
+  } else {
+    launch_gemm(d, 0, false, akc, bkc, stream, ws, prof);
   }
 }
 
