@@ -91,6 +91,19 @@ def roofline_from_profile(records):
     return roof, table
 
 
+def measured_traffic(kernel, config):
+    """HBM-side bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
+    this workload (profiles/gemm_f64_<config>_fp64_traffic.md), or None when there is none."""
+    path = os.path.join(ROOT, 'profiles', f'gemm_f64_{config}_fp64_traffic.md')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) == 3 and parts[0] == 'traffic_bytes_per_launch' and parts[1] == kernel:
+            return float(parts[2])
+    return None
+
+
 def cpu_baseline(system, k0, k1, max_pair_samples=2):
     """Time the CPU oracle (reference contraction order, FP64, OpenMP + MKL) on the
     same inputs: the opposite-spin part in full, the same-spin pair loops on a
@@ -134,7 +147,19 @@ def main():
     ap.add_argument('--kappa', default='1.0,0.5')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip'])
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='only copy 1 GiB device to device 3 times (reference for HBM byte counters) and exit')
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device='cuda')
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({'calibration': 'device copy', 'bytes_read': 1 << 30, 'bytes_written': 1 << 30,
+                          'launches': 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported by this build '
               '(the multi-GPU design shards the DF index and is documented, not run).', file=sys.stderr)
@@ -187,6 +212,9 @@ def main():
     ctx.set_profiling(False)
     value = args.steps / elapsed
     roof, table = roofline_from_profile(records)
+    roof['traffic'] = measured_traffic(roof['kernel'], args.config)
+    if roof['traffic'] is not None:
+        roof['traffic_source'] = f'profiles/gemm_f64_{args.config}_fp64_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)'
     gpu_flops = sum(r[2] for r in records) / args.steps
     model = flop_model(N, Nd, Nr, occ_counts, indirect)
     if args.profile_table:
