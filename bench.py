@@ -219,6 +219,22 @@ def main():
     model = flop_model(N, Nd, Nr, occ_counts, indirect)
     if args.profile_table:
         print(json.dumps(table, indent=1), file=sys.stderr)
+    # precision form on the same inputs (one untimed warm-up, one timed evaluation)
+    step32 = lambda: u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],  # noqa: E731
+                                 indirect_term=indirect, calculate_fock=True, scale_opp_spin=k0,
+                                 scale_same_spin=k1, precision=1)
+    step32()
+    sync()
+    t = time.perf_counter()
+    terms32, F32 = step32()
+    sync()
+    t32 = time.perf_counter() - t
+    fp32 = {'evals_per_s': round(1.0 / t32, 6), 'speedup_vs_fp64': round(elapsed / args.steps / t32, 3),
+            'term_error_eh': {k: terms32[k] - terms[k] for k in ('e2_direct', 'e2_indirect', 'e3_direct',
+                                                                 'e3_indirect', 'e4_direct', 'e4_indirect', 'total')},
+            'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(F32, F))}
+    print(f'[bench] FP32 form: {1e3 * t32:.1f} ms, total energy error {fp32["term_error_eh"]["total"]:.3e} Eh',
+          file=sys.stderr, flush=True)
     cpu = None
     if args.cpu_baseline == 'auto':
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
@@ -246,6 +262,7 @@ def main():
         'model_flops_per_eval': model,
         'executed_flops_per_eval': gpu_flops,
         'setup_s': round(t_setup, 2),
+        'fp32_form': fp32,
     }
     print(json.dumps(out), flush=True)
 

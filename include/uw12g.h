@@ -32,8 +32,11 @@ typedef struct uw12g_context uw12g_context;
 typedef struct uw12g_base_integrals uw12g_base_integrals;
 typedef struct uw12g_abs_projectors uw12g_abs_projectors;
 
-/* Arithmetic of the bulk contractions.  FP64 is the plain form. */
-typedef enum { UW12G_FP64 = 0 } uw12g_precision;
+/* Arithmetic of the bulk contractions.  FP64 is the plain form.  UW12G_FP32 runs
+ * every contraction of the evaluation on FP32 matrix cores (operands rounded to
+ * FP32, FP32 sums over k ranges of at most 4096, ranges combined in FP64);
+ * storage, the metric eigensolves and the energy reductions stay FP64. */
+typedef enum { UW12G_FP64 = 0, UW12G_FP32 = 1 } uw12g_precision;
 
 /* Term order of terms_out[] */
 enum {

@@ -20,12 +20,13 @@ typedef struct {
 } uw12g_map2;
 
 /* C <- alpha sum_k A(m,k) ks(k) B(k,n) + beta C for every batch, on device
- * `device`; host arrays are copied in and C is copied back.  naive != 0 runs
- * the reference kernel instead of the MFMA kernel; ks may be NULL. */
+ * `device`; host arrays are copied in and C is copied back.  mode 0 runs the
+ * FP64 MFMA kernel, 1 the one-thread-per-output FP64 kernel, 2 the FP32 MFMA
+ * form (operands rounded to FP32, FP32 sums per k chunk); ks may be NULL. */
 int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
                     size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, size_t nB,
                     uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,
-                    uw12g_map2 Cn, long long Cb, const double* ks, size_t nks, uw12g_map2 Km, int naive);
+                    uw12g_map2 Cn, long long Cb, const double* ks, size_t nks, uw12g_map2 Km, int mode);
 
 /* Average time in milliseconds of `reps` launches of the same GEMM on arrays
  * already in device memory (after one untimed launch), on a stream of its own. */

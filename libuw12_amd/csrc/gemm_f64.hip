@@ -19,11 +19,13 @@
 namespace uw12g {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 namespace {
 
 constexpr int BK = 16;
 constexpr int NT = 256;
+constexpr int kF32Chunk = 4096;  // longest FP32-accumulated k range (a multiple of BK)
 
 struct Params {
   GemmDesc d;
@@ -78,17 +80,24 @@ __device__ inline void slab_schedule() {
 // runs the k remainder separately) and the look-ahead past the last slab
 // re-reads it.  Other descriptors take a general path with 64-bit offsets and
 // clamped, zeroed loads.
-template <int BM, int BN, bool AKC, bool BKC, bool KS>
+// PREC selects the matrix-core arithmetic: 0 = FP64 (v_mfma_f64_16x16x4f64),
+// 1 = FP32 (v_mfma_f32_16x16x4f32: operands rounded to FP32 at the LDS write,
+// FP32 accumulation inside one launch chunk of at most kF32Chunk k values,
+// chunks combined in FP64 by the split-K reduction).  Storage stays FP64.
+template <int BM, int BN, bool AKC, bool BKC, bool KS, int PREC>
 __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
-  constexpr int PAD = 4;
+  using T = std::conditional_t<PREC == 0, double, float>;
+  using AccT = std::conditional_t<PREC == 0, d4, f4>;
+  // row padding: FP64 rows 4 doubles apart in banks; FP32 rows 16 floats apart
+  constexpr int PAD = PREC == 0 ? 4 : 16;
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int NA = BM * BK / NT;  // A elements loaded per thread and slab
   constexpr int NBL = BN * BK / NT;
   constexpr int G = 4;              // load groups per slab
   constexpr int GA = NA / G, GB = NBL / G;
-  __shared__ double As[2][BK][BM + PAD];
-  __shared__ double Bs[2][BK][BN + PAD];
+  __shared__ T As[2][BK][BM + PAD];
+  __shared__ T Bs[2][BK][BN + PAD];
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
@@ -133,28 +142,28 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     for (int i = q * GA; i < (q + 1) * GA; ++i) {
       double v = ra[i];
       if constexpr (KS) v *= rs[AKC ? q : i];
-      if constexpr (AKC) As[buf][a_kk0][t / BK + (NT / BK) * i] = v;
-      else As[buf][a_kk0 + (NT / BM) * i][t % BM] = v;
+      if constexpr (AKC) As[buf][a_kk0][t / BK + (NT / BK) * i] = T(v);
+      else As[buf][a_kk0 + (NT / BM) * i][t % BM] = T(v);
     }
 #pragma unroll
     for (int i = q * GB; i < (q + 1) * GB; ++i) {
-      if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = rb[i];
-      else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = rb[i];
+      if constexpr (BKC) Bs[buf][b_kk0][t / BK + (NT / BK) * i] = T(rb[i]);
+      else Bs[buf][b_kk0 + (NT / BN) * i][t % BN] = T(rb[i]);
     }
   };
 
-  d4 acc[MB][NB];
+  AccT acc[MB][NB];
 #pragma unroll
   for (int i = 0; i < MB; ++i)
 #pragma unroll
-    for (int j = 0; j < NB; ++j) acc[i][j] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < NB; ++j) acc[i][j] = AccT{0, 0, 0, 0};
 
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
 
   auto kstep = [&](auto qc, int buf) {
     constexpr int q = decltype(qc)::value;
-    double af[MB], bf[NB];
+    T af[MB], bf[NB];
 #pragma unroll
     for (int i = 0; i < MB; ++i) af[i] = As[buf][4 * q + lk][wm0 + 16 * i + lr];
 #pragma unroll
@@ -162,8 +171,12 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
 #pragma unroll
     for (int i = 0; i < MB; ++i)
 #pragma unroll
-      for (int j = 0; j < NB; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+      for (int j = 0; j < NB; ++j) {
+        if constexpr (PREC == 0)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(bf[j], af[i], acc[i][j], 0, 0, 0);
+        else
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(bf[j], af[i], acc[i][j], 0, 0, 0);
+      }
   };
   // Slab pipeline over [k_begin, k_end) with a group loader ld(q, k0); k0 of
   // a look-ahead slab may lie past the end (the loader clamps or masks it).
@@ -319,7 +332,10 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
 
   // ---- epilogue.  v_mfma_f64_16x16x4f64 leaves D[i][j] of its 16x16 block in
   // register i / 4 of lane j + 16 (i % 4) (one-row groups, four groups per
-  // block), so lane (lr, lk) register r holds C[m = .. + lr][n = .. + lk + 4 r].
+  // block), so lane (lr, lk) register r holds C[m = .. + lr][n = .. + lk + 4 r];
+  // v_mfma_f32_16x16x4f32 uses four-row groups: register i % 4 of lane
+  // j + 16 (i / 4), i.e. C[m = .. + lr][n = .. + 4 lk + r].
+  auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + (PREC == 0 ? lk + 4 * r : 4 * lk + r); };
   if (p.out) {
     double* W = p.out + (int64_t(blockIdx.y) * d.batch + b) * int64_t(d.M) * d.N;
 #pragma unroll
@@ -330,8 +346,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
       for (int j = 0; j < NB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = n0 + wn0 + 16 * j + lk + 4 * r;
-          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * acc[i][j][r];
+          const int n = ncol(j, r);
+          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * double(acc[i][j][r]);
         }
     }
     return;
@@ -346,10 +362,10 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     for (int j = 0; j < NB; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int n = n0 + wn0 + 16 * j + lk + 4 * r;
+        const int n = ncol(j, r);
         if (n >= d.N) continue;
         double* c = Cp + mo + d.Cn.off(n);
-        const double v = d.alpha * acc[i][j][r];
+        const double v = d.alpha * double(acc[i][j][r]);
         *c = d.beta == 0.0 ? v : v + d.beta * *c;
       }
   }
@@ -386,23 +402,28 @@ __global__ void gemm_naive_kernel(GemmDesc d) {
 }
 
 template <int BM, int BN>
-std::string cfg_name(bool akc, bool bkc, bool ks) {
-  return "gemm_f64_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
-         (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + ">";
+std::string cfg_name(bool akc, bool bkc, bool ks, int prec) {
+  return std::string(prec ? "gemm_f32_kernel<" : "gemm_f64_kernel<") + std::to_string(BM) + "," + std::to_string(BN) +
+         "," + (akc ? "1" : "0") + "," + (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + ">";
 }
 
-template <int BM, int BN, bool KS>
+template <int BM, int BN, bool KS, int PREC>
 void launch_ks(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
-  if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, KS>), grid, dim3(NT), 0, s, p);
-  else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, KS>), grid, dim3(NT), 0, s, p);
-  else if (bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, KS>), grid, dim3(NT), 0, s, p);
-  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, KS>), grid, dim3(NT), 0, s, p);
+  if (akc && bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, true, KS, PREC>), grid, dim3(NT), 0, s, p);
+  else if (akc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, true, false, KS, PREC>), grid, dim3(NT), 0, s, p);
+  else if (bkc) hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, true, KS, PREC>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, KS, PREC>), grid, dim3(NT), 0, s, p);
 }
 
 template <int BM, int BN>
 void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
-  if (p.d.ks) launch_ks<BM, BN, true>(p, akc, bkc, grid, s);
-  else launch_ks<BM, BN, false>(p, akc, bkc, grid, s);
+  if (p.d.prec == 1) {
+    if (p.d.ks) launch_ks<BM, BN, true, 1>(p, akc, bkc, grid, s);
+    else launch_ks<BM, BN, false, 1>(p, akc, bkc, grid, s);
+  } else {
+    if (p.d.ks) launch_ks<BM, BN, true, 0>(p, akc, bkc, grid, s);
+    else launch_ks<BM, BN, false, 0>(p, akc, bkc, grid, s);
+  }
 }
 
 // Unit stride on the low level of a map (contiguous runs along that index).
@@ -486,6 +507,12 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
     splits = std::max(1, std::min(splits, 64));
     while (splits > 1 && int64_t(splits) * d.batch > 65535) --splits;
   }
+  if (d.prec == 1 && klen > kF32Chunk) {
+    // FP32 accumulation runs over at most kF32Chunk k values; chunks add up in FP64
+    if (kbase != 0 || !ws) throw Error(-2, "gemm: FP32 k remainder longer than one chunk");
+    splits = std::max(splits, (klen + kF32Chunk - 1) / kF32Chunk);
+    if (splits > 65535) throw Error(-2, "gemm: too many FP32 k chunks");
+  }
   p.kchunk = ((kt + splits - 1) / splits) * BK;
   splits = (klen + p.kchunk - 1) / p.kchunk;
   p.splits = splits;
@@ -502,16 +529,16 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   std::string kname;
   if (BM == 128 && BN == 128) {
     launch_cfg<128, 128>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<128, 128>(akc, bkc, d.ks != nullptr);
+    if (timing) kname = cfg_name<128, 128>(akc, bkc, d.ks != nullptr, d.prec);
   } else if (BM == 128) {
     launch_cfg<128, 64>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<128, 64>(akc, bkc, d.ks != nullptr);
+    if (timing) kname = cfg_name<128, 64>(akc, bkc, d.ks != nullptr, d.prec);
   } else if (BN == 128) {
     launch_cfg<64, 128>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<64, 128>(akc, bkc, d.ks != nullptr);
+    if (timing) kname = cfg_name<64, 128>(akc, bkc, d.ks != nullptr, d.prec);
   } else {
     launch_cfg<64, 64>(p, akc, bkc, grid, stream);
-    if (timing) kname = cfg_name<64, 64>(akc, bkc, d.ks != nullptr);
+    if (timing) kname = cfg_name<64, 64>(akc, bkc, d.ks != nullptr, d.prec);
   }
   UW12G_HIP(hipGetLastError());
   if (timing) prof->end(rec, stream, kname, 2.0 * d.M * double(d.N) * klen * d.batch, 0.0);

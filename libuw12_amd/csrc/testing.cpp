@@ -67,8 +67,12 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
     d.ks = ks ? k.p : nullptr;
     d.Km = to_map(Km);
     Workspace ws;
-    if (naive) gemm_f64_naive(d, nullptr);
-    else gemm_f64(d, nullptr, &ws);
+    if (naive == 1) {
+      gemm_f64_naive(d, nullptr);
+    } else {
+      d.prec = naive == 2 ? 1 : 0;
+      gemm_f64(d, nullptr, &ws);
+    }
     UW12G_HIP(hipDeviceSynchronize());
     UW12G_HIP(hipMemcpy(C, c.p, sizeof(double) * nC, hipMemcpyDeviceToHost));
     return 0;

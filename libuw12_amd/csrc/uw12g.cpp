@@ -162,6 +162,7 @@ struct uw12g_context {
   // doubles allowed for one blocked intermediate of the indirect terms (K~,
   // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
   size_t block_cap = size_t(1) << 28;
+  int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
   Workspace gemm_ws;
@@ -170,6 +171,10 @@ struct uw12g_context {
 
   // Named device buffers reused across evaluations; growing or releasing one
   // first drains the stream so no queued kernel still reads the old memory.
+  // "blk1".."blk4" are scratch arenas shared by the consecutive blocked stages
+  // of the indirect terms (4e, 3e energy, 3e Fock i/j, 3e Fock k): each stage
+  // overwrites them in stream order, and keeping them allocated avoids a
+  // multi-GiB allocate/free per evaluation.
   double* buf(const std::string& name, size_t n) {
     auto& b = bufs[name];
     if (!b) b.reset(new DBuf());
@@ -229,11 +234,14 @@ void require(bool ok, const std::string& msg, int code = -1) {
 
 void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)); }
 
-// GEMM launch (timed per launch when profiling is on).
+// GEMM launch (timed per launch when profiling is on) in the context's
+// current contraction precision.
 void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   ctx->prof.stage = stage;
-  gemm_f64(d, ctx->stream, &ctx->gemm_ws, &ctx->prof);
+  GemmDesc e = d;
+  e.prec = ctx->prec;
+  gemm_f64(e, ctx->stream, &ctx->gemm_ws, &ctx->prof);
 }
 
 template <class F>
@@ -801,8 +809,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       scale_cols(q.o2V, ao, DV, V->sign.p, V2s, st);
       // ---- 4e indirect energy (four_electron.cpp:143-179, 274-278)
       {
-        double* W4 = ctx->buf("W4", size_t(ao) * ao);
-        double* V4 = ctx->buf("V4", size_t(ao) * ao);
+        double* W4 = ctx->buf("blk1", size_t(ao) * ao);
+        double* V4 = ctx->buf("blk2", size_t(ao) * ao);
         for (int op = 0; op < 2; ++op) {
           GemmDesc g;  // X4 = O2 diag(s) O2^T
           g.M = int(ao);
@@ -821,14 +829,12 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         }
         timed(ctx, "4e_ind_energy", "e4i_kernel", 2.0 * ao * ao, 16.0 * ao * ao,
               [&] { e4i_energy(W4, V4, a, o, -0.5 * e * k1, Eacc + UW12G_E4_INDIRECT, partial, st); });
-        ctx->release("W4");
-        ctx->release("V4");
       }
       // ---- 4e indirect Fock (four_electron.cpp:181-249)
       if (calc_fock) {
         const size_t nXY = size_t(N) * ao * a;
-        double* Xb = ctx->buf("f4_x", nXY);
-        double* Yb = ctx->buf("f4_y", nXY);
+        double* Xb = ctx->buf("blk1", nXY);
+        double* Yb = ctx->buf("blk2", nXY);
         GemmDesc gx;  // X_i[alpha, (j,k)] = sum_A Phi_W[alpha, c+i, A] sW_A O2_W[(j,k), A]
         gx.M = N;
         gx.N = int(ao);
@@ -882,8 +888,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         run_gemm(ctx, gf, "4e_ind_fock_f");
         if (c > 0) {
           const size_t nXY2 = size_t(N) * ao * o;
-          double* X2b = ctx->buf("f4_x2", nXY2);
-          double* Y2b = ctx->buf("f4_y2", nXY2);
+          double* X2b = ctx->buf("blk3", nXY2);
+          double* Y2b = ctx->buf("blk4", nXY2);
           GemmDesc hx = gx;  // X'_l for every occupied l
           hx.batch = o;
           hx.A = q.phiW;
@@ -911,19 +917,15 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           hf.A = X2b;
           hf.B = Y2b;
           run_gemm(ctx, hf, "4e_ind_fock_f");
-          ctx->release("f4_x2");
-          ctx->release("f4_y2");
         }
-        ctx->release("f4_x");
-        ctx->release("f4_y");
       }
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k
       {
         const int64_t Ma = int64_t(M) * a;
         const size_t cap = ctx->block_cap;
         int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
-        double* GWb = ctx->buf("e3_gw", size_t(Ma) * a * nk);
-        double* GVb = ctx->buf("e3_gv", size_t(Ma) * a * nk);
+        double* GWb = ctx->buf("blk1", size_t(Ma) * a * nk);
+        double* GVb = ctx->buf("blk2", size_t(Ma) * a * nk);
         for (int k0 = 0; k0 < o; k0 += nk) {
           const int kn = std::min(nk, o - k0);
           GemmDesc g;  // GW[(mu', i), (j, k)] = sum_A Phi_W[mu', c+i, A] W2s[(j, k), A]
@@ -950,8 +952,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           timed(ctx, "3e_ind_energy", "e3i_kernel", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
                 [&] { e3i_energy(GWb, GVb, M, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
         }
-        ctx->release("e3_gw");
-        ctx->release("e3_gv");
       }
       // ---- 3e indirect Fock (indirect_utils.cpp:138-345)
       if (calc_fock) {
@@ -970,14 +970,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         p.Cm = lin(1);
         p.Cn = lin(M);
         run_gemm(ctx, p, "3e_abs_apply");
-        double* L = ctx->buf("L", MN * std::max(DW, DV));
+        double* L = ctx->buf("blk2", MN * std::max(DW, DV));
         const int64_t MNo = int64_t(MN) * o;
         const size_t cap = ctx->block_cap;
         const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
-        double* Kt = ctx->buf("Kt", size_t(MNo) * nj);
+        double* Kt = ctx->buf("blk1", size_t(MNo) * nj);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
-        double* X2sT = ctx->buf("X2sT", size_t(ao) * std::max(DW, DV));
+        double* X2sT = ctx->buf("blk3", size_t(ao) * std::max(DW, DV));
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
@@ -1041,14 +1041,11 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           f.Cn = lin(N);
           run_gemm(ctx, f, "3e_ind_fock_f");
         }
-        ctx->release("L");
-        ctx->release("Kt");
-        ctx->release("X2sT");
         // fock_k (indirect_utils.cpp:199-259), blocked over j
         const int64_t MaN = int64_t(M) * a * N;
         const int nj2 = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MaN, 1))));
-        double* Qt = ctx->buf("Qt", size_t(MaN) * nj2);
-        double* R = ctx->buf("R", size_t(DW) * N * nj2);
+        double* Qt = ctx->buf("blk1", size_t(MaN) * nj2);
+        double* R = ctx->buf("blk2", size_t(DW) * N * nj2);
         for (int j0 = 0; j0 < a; j0 += nj2) {
           const int jn = std::min(nj2, a - j0);
           GemmDesc g;  // Q~[mu', i, alpha | j] = sum_B PhiT_V[mu', j, B] sV_B Phi_V[alpha, c+i, B]
@@ -1106,9 +1103,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           f.Cn = lin(N);
           run_gemm(ctx, f, "3e_ind_fock_k");
         }
-        ctx->release("Qt");
-        ctx->release("R");
-        ctx->release("phitW");
       }
     }
   }
@@ -1276,9 +1270,15 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision == UW12G_FP64, "unsupported precision", -2);
+    require(precision == UW12G_FP64 || precision == UW12G_FP32, "unsupported precision", -2);
+    require(ctx != nullptr, "null context");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
+    struct PrecScope {  // the setup paths always run in FP64
+      uw12g_context* c;
+      ~PrecScope() { c->prec = 0; }
+    } scope{ctx};
+    ctx->prec = precision == UW12G_FP32 ? 1 : 0;
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
              scale_opp_spin, scale_same_spin, fock, tp, energy);
   });

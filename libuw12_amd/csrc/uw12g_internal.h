@@ -82,6 +82,7 @@ struct GemmDesc {
   const double* ks = nullptr;
   Map2 Km;
   int64_t Kb = 0;
+  int prec = 0;  // matrix-core arithmetic: 0 FP64, 1 FP32 (operands and chunk sums in FP32, FP64 storage)
 };
 
 // Allocation registry: every device buffer the library allocates is recorded

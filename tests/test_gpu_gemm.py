@@ -1,4 +1,4 @@
-"""The strided FP64 MFMA GEMM kernel against the plain kernel and numpy."""
+"""The strided MFMA GEMM kernel (FP64 and FP32 forms) against the plain kernel and numpy."""
 import numpy as np
 import pytest
 
@@ -93,3 +93,83 @@ def test_zero_k_scales_c():
     B = np.zeros(1)
     out = run(3, 4, 0, 1, 1.0, 2.0, A, M2(1), M2(3), 0, B, M2(1), M2(1), 0, C, M2(1), M2(3), 0)
     assert np.array_equal(out, 2 * C)
+
+
+def fp32_bound(M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, ks=None, Km=None, chunk=4096):
+    """Elementwise bound on |C_fp32 - C_exact| for the FP32 form: operands rounded
+    to FP32 (relative 2^-24 each) and FP32 recursive summation over k chunks of at
+    most `chunk` (gamma_n = n u / (1 - n u), Higham, Accuracy and Stability of
+    Numerical Algorithms, 2nd ed., eq. 3.5), chunks added in FP64."""
+    u = 2.0 ** -24
+    n = min(K, chunk) + 2
+    g = n * u / (1 - n * u)
+    absC = host_gemm(M, N, K, batch, abs(alpha), 0.0, np.abs(A), Am, Ak, Ab, np.abs(B), Bk, Bn, Bb,
+                     np.zeros(_c_size(M, N, batch, Cm, Cn, Cb)), Cm, Cn, Cb,
+                     None if ks is None else np.abs(ks), Km)
+    return g * absC
+
+
+def _c_size(M, N, batch, Cm, Cn, Cb):
+    def mx(m, e):
+        if e <= 0:
+            return 0
+        last = e - 1
+        if m.n0 == 0:
+            return last * m.s0
+        q, r = divmod(last, m.n0)
+        best = r * m.s0 + q * m.s1
+        if q >= 1:
+            best = max(best, (m.n0 - 1) * m.s0 + (q - 1) * m.s1)
+        return best
+    return Cb * (batch - 1) + mx(Cm, M) + mx(Cn, N) + 1
+
+
+@pytest.mark.parametrize('name,M,N,K,batch', CASES)
+@pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
+def test_fp32_form_within_rounding_bound(name, M, N, K, batch, layout):
+    rng = np.random.default_rng(hash(('fp32', name, layout)) % 2**32)
+    A = rng.standard_normal(M * K * batch)
+    B = rng.standard_normal(K * N * batch)
+    C = np.zeros(M * N * batch)
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    args = (M, N, K, batch, 1.25, 0.0, A, Am, Ak, M * K, B, Bk, Bn, K * N, C, M2(1), M2(M), M * N)
+    ref = host_gemm(*args)
+    got = run(*args, naive=2)
+    bound = fp32_bound(M, N, K, batch, 1.25, A, Am, Ak, M * K, B, Bk, Bn, K * N, M2(1), M2(M), M * N)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound + 1e-300), (err / np.maximum(bound, 1e-300)).max()
+    # and it is really FP32: not bit-equal to the FP64 result on a generic input
+    assert np.abs(got - ref).max() > 0
+
+
+@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
+@pytest.mark.parametrize('kc', [False, True])
+def test_affine_two_level_k_maps(mode, K, kc):
+    """Compound k = (r, q) with an inner period of 32 (a multiple of the 16-deep
+    slab): the unpredicated buffer-load path, in both the k-along-waves form
+    (kc False) and the k-along-lanes form (kc True, where a k remainder runs as
+    a second launch), with a k scale."""
+    rng = np.random.default_rng(17 + K + 2 * kc + mode)
+    M, N, n0 = 200, 150, 32
+    q = (K + n0 - 1) // n0
+    # A(m, k) = A[m*sm + (k % n0)*s0 + (k // n0)*s1]
+    if kc:
+        Am, Ak = M2(n0 * q + 3), M2(1, n0, n0)
+    else:
+        Am, Ak = M2(1), M2(M + 5, (M + 5) * n0 + 11, n0)
+    A = rng.standard_normal(M * (n0 * q + 3) + (M + 5) * n0 * q + 11 * q + 1000)
+    Bk, Bn = (M2(1, n0 + 0, n0), M2(n0 * q + 1)) if kc else (M2(N, N * n0 + 3, n0), M2(1))
+    B = rng.standard_normal(N * (n0 * q + 1) + N * n0 * q + 3 * q + 1000)
+    ks = rng.standard_normal(n0 * q + 10)
+    Km = M2(1, n0, n0)
+    C = rng.standard_normal(M * N)
+    args = (M, N, K, 1, 0.75, 0.5 if mode == 0 else 0.0, A, Am, Ak, 0, B, Bk, Bn, 0, C, M2(1), M2(M), 0, ks, Km)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode)
+    if mode == 0:
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    else:
+        bound = fp32_bound(M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, Km)
+        assert np.all(np.abs(got - ref) <= bound)

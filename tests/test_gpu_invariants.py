@@ -206,3 +206,27 @@ def test_water_integrals(gpu_ctx):
         for k in TERMS + ('total',):
             assert abs(eg[k] - eo[k]) <= 1e-9 * max(1.0, abs(eo[k])), (spin, k, eg[k], eo[k])
         assert fock_error(Fg, Fo) <= 1e-8 * max(1.0, max(np.abs(f).max() for f in Fo))
+
+
+def test_fp32_form_errors(gpu_ctx):
+    """FP32 form against FP64 on the c1 recipe.  The two- and four-electron terms
+    (no RI projector) agree to FP32 rounding of the contractions; the
+    three-electron terms carry S+ (norm ~1e5 here: kept RI-overlap eigenvalues
+    down to ~1e-6), whose cancellation amplifies FP32 rounding by ~1e4, so only
+    their finiteness and the order of magnitude of the deviation are checked."""
+    print('case c1 fp32 form', flush=True)
+    s = synth.make_system('c1')
+    h = gpu_setup(s, gpu_ctx)
+    W, V, WV, P = h
+    e64, F64 = u.form_fock(W, V, WV, P, s['orbitals'], s['occ'], s['n_active'], indirect_term=True,
+                           scale_opp_spin=1.0, scale_same_spin=0.5, precision=0)
+    e32, F32 = u.form_fock(W, V, WV, P, s['orbitals'], s['occ'], s['n_active'], indirect_term=True,
+                           scale_opp_spin=1.0, scale_same_spin=0.5, precision=1)
+    for k in ('e2_direct', 'e2_indirect', 'e4_direct', 'e4_indirect'):
+        assert abs(e32[k] - e64[k]) <= 1e-5 * max(1.0, abs(e64[k])), (k, e32[k], e64[k])
+        assert e32[k] != e64[k]
+    for k in ('e3_direct', 'e3_indirect', 'total'):
+        assert np.isfinite(e32[k]) and abs(e32[k] - e64[k]) <= 1e-1 * max(1.0, abs(e64[k])), (k, e32[k], e64[k])
+    assert all(np.isfinite(f).all() for f in F32)
+    with pytest.raises(u.UW12Error):
+        u.form_fock(W, V, WV, P, s['orbitals'], s['occ'], s['n_active'], precision=7)
