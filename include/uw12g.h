@@ -104,6 +104,26 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy);
 
+/* Differentiation form, forward half: the energy and its six pieces as
+ * uw12g_form_fock (fock == NULL), keeping the forward intermediates (the W/V
+ * orbital transforms, (P|ij), t_AB, S+ Phi_V, X_AB) on the context.  *tape
+ * identifies them for uw12g_adjoint.  No reference counterpart: the reference
+ * computes energy and Fock in one pass (src/uw12.cpp:22-119). */
+int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                  const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                  size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                  int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
+                  double* terms_out, double* energy, unsigned long long* tape);
+
+/* Differentiation form, adjoint half: F = dE/dD (n_ao x n_ao per spin, alpha
+ * then beta, symmetric) of the forward call `tape`, from its kept
+ * intermediates; the result equals uw12g_form_fock's Fock on the same inputs
+ * (all-electron: the exact derivative; frozen core: the reference's averaged
+ * form, four_electron.cpp:63-76).  Returns -4 when a later fused or forward
+ * evaluation on the context has replaced the intermediates.  The integral and
+ * projector handles of the forward call must still exist. */
+int uw12g_adjoint(uw12g_context* ctx, unsigned long long tape, double* fock);
+
 /* Kernel-level timing of the most recent evaluation on this context:
  * names[i] / milliseconds[i] for up to cap entries; returns the count.
  * Enabled by uw12g_context_set_profiling(ctx, 1).  No reference counterpart. */

@@ -22,7 +22,8 @@ EXPORTS = (
     'uw12g_last_error', 'uw12g_context_create', 'uw12g_context_destroy', 'uw12g_setup_base_integrals',
     'uw12g_setup_base_integrals_device', 'uw12g_base_integrals_destroy', 'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
-    'uw12g_fock', 'uw12g_form_fock', 'uw12g_context_set_profiling', 'uw12g_context_profile',
+    'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
+    'uw12g_context_profile',
     'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm',
 )
 
@@ -59,6 +60,9 @@ def load():
     L.uw12g_fock.argtypes = [vp, dp] + common + [ctypes.c_double, ctypes.c_double, dp]
     L.uw12g_form_fock.argtypes = [vp] + common + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
                                                   dp, dp]
+    L.uw12g_forward.argtypes = [vp] + common + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp,
+                                                ctypes.POINTER(ctypes.c_ulonglong)]
+    L.uw12g_adjoint.argtypes = [vp, ctypes.c_ulonglong, dp]
     L.uw12g_context_set_profiling.argtypes = [vp, ctypes.c_int]
     L.uw12g_context_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, dp, ctypes.c_int]
     L.uw12g_test_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -231,6 +231,37 @@ def form_fock(W, V, WV, abs_projectors, orbitals, occ, n_active, indirect_term=T
     return out, fock
 
 
+class Tape:
+    """Handle of a forward call's intermediates (differentiation form)."""
+
+    def __init__(self, ctx, tape_id, n_ao, n_spin):
+        self.ctx, self.id, self.n_ao, self.n_spin = ctx, tape_id, n_ao, n_spin
+
+
+def forward(W, V, WV, abs_projectors, orbitals, occ, n_active, indirect_term=True, scale_opp_spin=1.0,
+            scale_same_spin=0.0, precision=0):
+    """Differentiation form, forward half (uw12g_forward): returns (energy pieces as form_fock, Tape)."""
+    C, n_ao, n_orb, n_spin, occv, n_occ, n_act = _orbital_args(orbitals, occ, n_active)
+    ctx = W.ctx
+    terms = np.zeros(6)
+    energy = ctypes.c_double(0.0)
+    tape = ctypes.c_ulonglong(0)
+    check(_lib.load().uw12g_forward(ctx._h, W._h, V._h, WV._h, abs_projectors._h, _ptr(C), n_ao, n_orb, n_spin,
+                                    _ptr(occv), n_occ, n_act, 1 if indirect_term else 0, float(scale_opp_spin),
+                                    float(scale_same_spin), int(precision), _ptr(terms), ctypes.byref(energy),
+                                    ctypes.byref(tape)))
+    out = dict(zip(TERMS, terms.tolist()))
+    out['total'] = energy.value
+    return out, Tape(ctx, tape.value, n_ao, n_spin)
+
+
+def adjoint(tape):
+    """Differentiation form, adjoint half (uw12g_adjoint): F = dE/dD per spin from the forward's intermediates."""
+    F = np.zeros((tape.n_ao, tape.n_ao * tape.n_spin), order='F')
+    check(_lib.load().uw12g_adjoint(tape.ctx._h, ctypes.c_ulonglong(tape.id), _ptr(F)))
+    return [np.array(F[:, s * tape.n_ao:(s + 1) * tape.n_ao]) for s in range(tape.n_spin)]
+
+
 def uw12_energy(W, V, WV, abs_projectors, orbitals, occ, n_active, scale_opp_spin, scale_same_spin,
                 print_level=0):
     """libuw12.hpp:129-140."""

@@ -163,6 +163,21 @@ struct uw12g_context {
   // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
   size_t block_cap = size_t(1) << 28;
   int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
+  // Differentiation form: arguments of the last forward call.  Its
+  // intermediates live in the named buffers until the next fused or forward
+  // evaluation on this context (n_eval counts those).
+  struct Tape {
+    uint64_t id = 0, n_eval = 0;
+    const uw12g_base_integrals *W = nullptr, *V = nullptr, *WV = nullptr;
+    const uw12g_abs_projectors* P = nullptr;
+    std::vector<double> C, occ;
+    std::vector<size_t> n_occ, n_active;
+    size_t n_ao = 0, n_orb = 0, n_spin = 0;
+    bool indirect = false;
+    double k0 = 0.0, k1 = 0.0;
+    int prec = 0;
+  } tape;
+  uint64_t n_eval = 0, n_tapes = 0;
   hipStream_t stream = nullptr;
   rocblas_handle rb = nullptr;
   Workspace gemm_ws;
@@ -405,10 +420,18 @@ double spin_factor(int ns, int s, int t, double k0, double k1) {  // four_electr
   return s == t ? k1 : k0;
 }
 
+// What one evaluate() call does: the fused energy (+ optional Fock) pass, the
+// forward half of the differentiation form (energy only; the W/V transforms
+// Phi, O2, t, S+ Phi_V and X stay in the context's named buffers), or its
+// adjoint half (Fock only, reading those buffers instead of recomputing them
+// and skipping the energy-only contractions of the indirect terms).
+enum EvalMode { kFused = 0, kForward = 1, kAdjoint = 2 };
+
 void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
               const uw12g_base_integrals* WV, const uw12g_abs_projectors* P, const double* C, size_t n_ao,
               size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
-              bool indirect_term, double k0, double k1, double* fock, double* terms, double* energy) {
+              bool indirect_term, double k0, double k1, double* fock, double* terms, double* energy,
+              EvalMode mode = kFused) {
   require(ctx && W && V && WV && P, "null handle");
   require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
   require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");
@@ -596,6 +619,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     q.t = ctx->buf("t" + sfx, size_t(DW) * DV);
     q.x = ctx->buf("x" + sfx, size_t(DW) * DV);
     q.phitV = ctx->buf("phitV" + sfx, size_t(M) * a * DV);
+    if (mode == kAdjoint) continue;  // kept from the forward call
     const double* Ca = q.Co + size_t(N) * c;
     for (int op = 0; op < 2; ++op) {
       const uw12g_base_integrals* X = op == 0 ? W : V;
@@ -808,7 +832,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       scale_cols(q.o2W, ao, DW, W->sign.p, W2s, st);
       scale_cols(q.o2V, ao, DV, V->sign.p, V2s, st);
       // ---- 4e indirect energy (four_electron.cpp:143-179, 274-278)
-      {
+      if (mode != kAdjoint) {
         double* W4 = ctx->buf("blk1", size_t(ao) * ao);
         double* V4 = ctx->buf("blk2", size_t(ao) * ao);
         for (int op = 0; op < 2; ++op) {
@@ -920,7 +944,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         }
       }
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k
-      {
+      if (mode != kAdjoint) {
         const int64_t Ma = int64_t(M) * a;
         const size_t cap = ctx->block_cap;
         int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
@@ -1264,6 +1288,15 @@ int uw12g_abs_projectors_get(const uw12g_abs_projectors* abs, double* out) {
   });
 }
 
+namespace {
+// Sets the contraction precision for one evaluation (the setup paths always run in FP64).
+struct PrecScope {
+  uw12g_context* c;
+  PrecScope(uw12g_context* ctx, int prec) : c(ctx) { c->prec = prec; }
+  ~PrecScope() { c->prec = 0; }
+};
+}  // namespace
+
 int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
                     const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
                     size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
@@ -1274,13 +1307,64 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
     require(ctx != nullptr, "null context");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
-    struct PrecScope {  // the setup paths always run in FP64
-      uw12g_context* c;
-      ~PrecScope() { c->prec = 0; }
-    } scope{ctx};
-    ctx->prec = precision == UW12G_FP32 ? 1 : 0;
+    PrecScope scope(ctx, precision == UW12G_FP32 ? 1 : 0);
+    ++ctx->n_eval;
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
              scale_opp_spin, scale_same_spin, fock, tp, energy);
+  });
+}
+
+int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                  const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C, size_t n_ao,
+                  size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
+                  int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
+                  double* terms_out, double* energy, unsigned long long* tape) {
+  return guard([&] {
+    require(precision == UW12G_FP64 || precision == UW12G_FP32, "unsupported precision", -2);
+    require(ctx != nullptr && tape != nullptr, "null argument");
+    require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
+    require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");
+    double t6[6] = {0, 0, 0, 0, 0, 0};
+    double* tp = terms_out ? terms_out : t6;
+    PrecScope scope(ctx, precision == UW12G_FP32 ? 1 : 0);
+    ++ctx->n_eval;
+    ctx->tape.id = 0;  // no valid tape while the forward runs
+    evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
+             scale_opp_spin, scale_same_spin, nullptr, tp, energy, kForward);
+    uw12g_context::Tape& t = ctx->tape;
+    size_t n_occ_tot = 0;
+    for (size_t s = 0; s < n_spin; ++s) n_occ_tot += n_occ[s];
+    t.W = W;
+    t.V = V;
+    t.WV = WV;
+    t.P = abs;
+    t.C.assign(C, C + n_ao * n_orb * n_spin);
+    t.occ.assign(occ, occ + n_occ_tot);
+    t.n_occ.assign(n_occ, n_occ + n_spin);
+    t.n_active.assign(n_active, n_active + n_spin);
+    t.n_ao = n_ao;
+    t.n_orb = n_orb;
+    t.n_spin = n_spin;
+    t.indirect = indirect_term != 0;
+    t.k0 = scale_opp_spin;
+    t.k1 = scale_same_spin;
+    t.prec = ctx->prec;
+    t.n_eval = ctx->n_eval;
+    t.id = ++ctx->n_tapes;
+    *tape = t.id;
+  });
+}
+
+int uw12g_adjoint(uw12g_context* ctx, unsigned long long tape, double* fock) {
+  return guard([&] {
+    require(ctx != nullptr && fock != nullptr, "null argument");
+    const uw12g_context::Tape& t = ctx->tape;
+    require(tape != 0 && tape == t.id, "unknown or superseded forward tape", -4);
+    require(t.n_eval == ctx->n_eval, "the forward intermediates were overwritten by a later evaluation", -4);
+    PrecScope scope(ctx, t.prec);
+    double t6[6];
+    evaluate(ctx, t.W, t.V, t.WV, t.P, t.C.data(), t.n_ao, t.n_orb, t.n_spin, t.occ.data(), t.n_occ.data(),
+             t.n_active.data(), t.indirect, t.k0, t.k1, fock, t6, nullptr, kAdjoint);
   });
 }
 
