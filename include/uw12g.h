@@ -68,6 +68,24 @@ int uw12g_setup_base_integrals(uw12g_context* ctx, const double* X3, const doubl
 int uw12g_setup_base_integrals_device(uw12g_context* ctx, const double* X3_dev, const double* X2,
                                       const double* X3_ri_dev, size_t n_ao, size_t n_df, size_t n_ri,
                                       uw12g_base_integrals** out);
+
+/* Three-index integrals of one DF shell, for uw12g_setup_base_integrals_fn:
+ * write the column-major n_rows x df_sizes[shell] block of shell `shell` to
+ * `out` (n_rows = n_ao (n_ao + 1) / 2 packed AO pairs, row mu (mu + 1) / 2 + nu;
+ * or n_ao n_ri, row r + n_ri mu, for the RI callback) and return 0. */
+typedef int (*uw12g_three_index_fn)(void* user, size_t shell, double* out);
+
+/* Replaces the reference's function-based BaseIntegrals constructors
+ * (base_integrals.hpp:52-72 with integral_functions.hpp:16-55): the
+ * three-index integrals are requested one DF shell at a time and streamed to
+ * the device in groups of whole shells, so the caller never holds X3 / X3_ri
+ * whole.  X2 is the two-index matrix (the TwoIndexFn result); three_index_ri
+ * NULL with n_ri = 0 gives the no-RI form (:64-72).  A callback returning
+ * nonzero fails the call with -6. */
+int uw12g_setup_base_integrals_fn(uw12g_context* ctx, const double* X2, uw12g_three_index_fn three_index,
+                                  uw12g_three_index_fn three_index_ri, void* user, const size_t* df_sizes,
+                                  size_t n_shells, size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out);
+
 void uw12g_base_integrals_destroy(uw12g_base_integrals* ints);
 /* Number of DF metric eigenvectors kept (|lambda| >= 1e-10), or negative status. */
 long uw12g_base_integrals_kept(const uw12g_base_integrals* ints);

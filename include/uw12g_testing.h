@@ -10,6 +10,8 @@
 
 #include <stddef.h>
 
+#include "uw12g.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,6 +36,10 @@ int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, do
                     uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, uw12g_map2 Bk, uw12g_map2 Bn,
                     long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks,
                     uw12g_map2 Km, int reps, double* ms);
+
+/* Host staging size (doubles per buffer) of uw12g_setup_base_integrals_fn on
+ * this context; small values force many shell groups (tests). */
+int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles);
 
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);

@@ -20,12 +20,17 @@ class Map2(ctypes.Structure):
 # every symbol declared in include/uw12g.h and include/uw12g_testing.h
 EXPORTS = (
     'uw12g_last_error', 'uw12g_context_create', 'uw12g_context_destroy', 'uw12g_setup_base_integrals',
-    'uw12g_setup_base_integrals_device', 'uw12g_base_integrals_destroy', 'uw12g_base_integrals_kept',
+    'uw12g_setup_base_integrals_device', 'uw12g_setup_base_integrals_fn', 'uw12g_base_integrals_destroy',
+    'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
     'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
     'uw12g_context_profile',
-    'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm',
+    'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm', 'uw12g_test_set_stream_cap',
 )
+
+
+# int (*)(void* user, size_t shell, double* out): one DF shell of three-index integrals
+THREE_INDEX_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double))
 
 
 class LibraryMissing(RuntimeError):
@@ -47,6 +52,9 @@ def load():
                                              ctypes.POINTER(vp)]
     L.uw12g_setup_base_integrals_device.argtypes = [vp, vp, dp, vp, ctypes.c_size_t, ctypes.c_size_t,
                                                     ctypes.c_size_t, ctypes.POINTER(vp)]
+    L.uw12g_setup_base_integrals_fn.argtypes = [vp, dp, THREE_INDEX_FN, THREE_INDEX_FN, vp,
+                                                ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t, ctypes.c_size_t,
+                                                ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(vp)]
     L.uw12g_base_integrals_destroy.argtypes = [vp]
     L.uw12g_base_integrals_destroy.restype = None
     L.uw12g_base_integrals_kept.argtypes = [vp]
@@ -77,6 +85,7 @@ def load():
                                   vp, Map2, Map2, ctypes.c_longlong,
                                   vp, Map2, Map2, ctypes.c_longlong,
                                   vp, Map2, ctypes.c_int, dp]
+    L.uw12g_test_set_stream_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]
     L.uw12g_test_map2.restype = ctypes.c_longlong
     _lib = L

@@ -158,6 +158,47 @@ def setup_base_integrals_device(X3_dev, X2, X3_ri_dev, n_ao, n_df, n_ri, context
     return BaseIntegrals(h, ctx, n_ao, n_df, n_ri)
 
 
+THREE_INDEX_FN = _lib.THREE_INDEX_FN
+
+
+def setup_base_integrals_fn(X2, three_index, three_index_ri, df_sizes, n_ao, n_ri=0, context=None):
+    """Function-based form (base_integrals.hpp:52-72): three_index(shell) returns the
+    (n_ao(n_ao+1)/2, df_sizes[shell]) block of DF shell `shell`, three_index_ri(shell)
+    the (n_ao n_ri, df_sizes[shell]) RI block (or None with n_ri = 0).  The library
+    requests the shells in groups and streams them to the device."""
+    ctx = context or Context.default()
+    X2 = _f64(X2)
+    n_df = X2.shape[0]
+    sizes = [int(x) for x in df_sizes]
+    P = n_ao * (n_ao + 1) // 2
+    errors = []
+
+    def wrap(fn, rows):
+        def cb(_user, shell, out):
+            try:
+                blk = np.asarray(fn(int(shell)), dtype=np.float64)
+                if blk.shape != (rows, sizes[shell]):
+                    raise ValueError(f'shell {shell}: block shape {blk.shape}, expected {(rows, sizes[shell])}')
+                dst = np.ctypeslib.as_array(out, shape=(rows * sizes[shell],))
+                dst[:] = blk.reshape(-1, order='F')
+                return 0
+            except Exception as exc:  # reported through the library's error path
+                errors.append(exc)
+                return 1
+        return THREE_INDEX_FN(cb)
+
+    f3 = wrap(three_index, P)
+    f3ri = wrap(three_index_ri, n_ao * n_ri) if three_index_ri is not None else THREE_INDEX_FN()
+    sz = (ctypes.c_size_t * len(sizes))(*sizes)
+    h = vp()
+    rc = _lib.load().uw12g_setup_base_integrals_fn(ctx._h, _ptr(X2), f3, f3ri, None, sz, len(sizes), n_ao, n_df,
+                                                   n_ri if three_index_ri is not None else 0, ctypes.byref(h))
+    if rc != 0 and errors:
+        raise UW12Error(rc, f'{_lib.load().uw12g_last_error().decode()}: {errors[0]}')
+    check(rc)
+    return BaseIntegrals(h, ctx, n_ao, n_df, n_ri if three_index_ri is not None else 0)
+
+
 def setup_abs_projectors(S, n_ao, n_ri, context=None):
     """libuw12.hpp:80."""
     ctx = context or Context.default()

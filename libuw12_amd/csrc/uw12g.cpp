@@ -163,6 +163,7 @@ struct uw12g_context {
   // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
   size_t block_cap = size_t(1) << 28;
   int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
+  size_t stream_cap = size_t(1) << 25;  // doubles per host staging buffer of the per-shell setup
   // Differentiation form: arguments of the last forward call.  Its
   // intermediates live in the named buffers until the next fused or forward
   // evaluation on this context (n_eval counts those).
@@ -303,15 +304,49 @@ bool is_symmetric(const double* X, size_t n, double tol) {
   return true;
 }
 
-void setup_base(uw12g_context* ctx, const double* X3, const double* X2, const double* X3ri, size_t n_ao,
-                size_t n_df, size_t n_ri, bool device_ptrs, uw12g_base_integrals** out) {
+// Source of the three-index input, one block of DF columns at a time: host
+// arrays, device arrays, or per-shell callbacks (base_integrals.hpp:52-72,
+// integral_functions.hpp:16-55).  For callbacks, blocks are groups of whole
+// shells.
+struct ThreeIndexSource {
+  const double* X3 = nullptr;  // packed AO rows x n_df (host or device)
+  const double* X3ri = nullptr;
+  bool device = false;
+  uw12g_three_index_fn fn = nullptr, fn_ri = nullptr;
+  void* user = nullptr;
+  std::vector<size_t> shells;  // DF shell sizes (callbacks)
+};
+
+struct PinnedBuf {  // page-locked host staging (hipHostMalloc)
+  double* p = nullptr;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void get(size_t n) {
+    UW12G_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), sizeof(double) * std::max<size_t>(n, 1), hipHostMallocDefault));
+  }
+};
+
+void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X2, size_t n_ao, size_t n_df,
+                size_t n_ri, uw12g_base_integrals** out) {
   require(ctx != nullptr, "null context");
   require(out != nullptr, "null output handle");
-  require(X3 != nullptr && X2 != nullptr, "null integral array");
+  require(X2 != nullptr, "null integral array");
+  const bool cb = src.fn != nullptr;
+  require(cb || src.X3 != nullptr, "null integral array");
   require(n_ao > 0, "Number of ao orbitals is zero");
   require(n_df > 0, "Number of density-fitting basis functions is zero");
-  require((X3ri == nullptr) == (n_ri == 0), "X3_ri and n_ri must both be given or both be absent");
+  if (cb) require((src.fn_ri == nullptr) == (n_ri == 0), "three_index_ri and n_ri must both be given or both be absent");
+  else require((src.X3ri == nullptr) == (n_ri == 0), "X3_ri and n_ri must both be given or both be absent");
   require(n_ao < 46341 && n_df < (1u << 30) && n_ri < (1u << 30), "size out of range", -2);
+  if (cb) {
+    size_t tot = 0;
+    for (size_t sz : src.shells) {
+      require(sz > 0, "empty density-fitting shell");
+      tot += sz;
+    }
+    require(tot == n_df, "density-fitting shell sizes do not add up to n_df");
+  }
   require(is_symmetric(X2, n_df, 1e-10), "matrix must be symmetric");
   set_device(ctx);
   std::unique_ptr<uw12g_base_integrals> I(new uw12g_base_integrals());
@@ -360,22 +395,71 @@ void setup_base(uw12g_context* ctx, const double* X3, const double* X2, const do
     *out = I.release();
     return;
   }
-  const size_t P = size_t(N) * (N + 1) / 2;
-  const size_t budget = size_t(1) << 28;  // doubles per staging block (2 GiB)
+  const size_t P = size_t(N) * (N + 1) / 2, PR = size_t(N) * Nr;
+  // column blocks [A0, A0 + na): at most `budget` doubles of G0 per block; with
+  // callbacks whole shells, and the host staging capped at 2^25 doubles per buffer
+  const size_t budget = size_t(1) << 28;
   int cA = int(std::max<size_t>(1, std::min<size_t>(Nd, budget / std::max<size_t>(MN, 1))));
-  DBuf G0, st3, stri;
-  G0.get(MN * cA);
-  if (!device_ptrs) {
-    st3.get(P * cA);
-    if (Nr > 0) stri.get(size_t(N) * Nr * cA);
+  std::vector<std::pair<int, int>> blocks;  // (A0, na)
+  std::vector<size_t> shell_first;           // callbacks: first shell of each block
+  if (cb) {
+    const size_t cap = std::max<size_t>(1, ctx->stream_cap / std::max<size_t>(P + PR, 1));
+    cA = int(std::min<size_t>(size_t(cA), cap));
+    size_t A = 0, sh = 0;
+    while (sh < src.shells.size()) {
+      const size_t first = sh, A0 = A;
+      size_t na = 0;
+      do {
+        na += src.shells[sh];
+        A += src.shells[sh];
+        ++sh;
+      } while (sh < src.shells.size() && na + src.shells[sh] <= size_t(cA));
+      blocks.emplace_back(int(A0), int(na));
+      shell_first.push_back(first);
+    }
+  } else {
+    for (int A0 = 0; A0 < Nd; A0 += cA) blocks.emplace_back(A0, std::min(cA, Nd - A0));
   }
-  for (int A0 = 0; A0 < Nd; A0 += cA) {
-    const int na = std::min(cA, Nd - A0);
-    const double* x3 = X3 + P * A0;
-    const double* xri = Nr > 0 ? X3ri + size_t(N) * Nr * A0 : nullptr;
-    if (!device_ptrs) {
-      UW12G_HIP(hipMemcpyAsync(st3.p, x3, sizeof(double) * P * na, hipMemcpyHostToDevice, s));
-      if (Nr > 0) UW12G_HIP(hipMemcpyAsync(stri.p, xri, sizeof(double) * size_t(N) * Nr * na, hipMemcpyHostToDevice, s));
+  int wmax = 0;
+  for (auto& b : blocks) wmax = std::max(wmax, b.second);
+  DBuf G0, st3, stri;
+  G0.get(MN * wmax);
+  if (!src.device) {
+    st3.get(P * wmax);
+    if (Nr > 0) stri.get(PR * wmax);
+  }
+  PinnedBuf h3, hri;  // callbacks fill these; each block's copy completes before the next fill
+  if (cb) {
+    h3.get(P * wmax);
+    if (Nr > 0) hri.get(PR * wmax);
+  }
+  for (size_t bi = 0; bi < blocks.size(); ++bi) {
+    const int A0 = blocks[bi].first, na = blocks[bi].second;
+    const double* x3 = nullptr;
+    const double* xri = nullptr;
+    if (src.device) {
+      x3 = src.X3 + P * A0;
+      xri = Nr > 0 ? src.X3ri + PR * A0 : nullptr;
+    } else if (!cb) {
+      UW12G_HIP(hipMemcpyAsync(st3.p, src.X3 + P * A0, sizeof(double) * P * na, hipMemcpyHostToDevice, s));
+      if (Nr > 0)
+        UW12G_HIP(hipMemcpyAsync(stri.p, src.X3ri + PR * A0, sizeof(double) * PR * na, hipMemcpyHostToDevice, s));
+      x3 = st3.p;
+      xri = stri.p;
+    } else {
+      UW12G_HIP(hipStreamSynchronize(s));  // the previous block's copies have drained the staging
+      size_t col = 0;
+      for (size_t sh = shell_first[bi]; col < size_t(na); ++sh) {
+        const int rc = src.fn(src.user, sh, h3.p + P * col);
+        require(rc == 0, "three_index callback failed for shell " + std::to_string(sh), -6);
+        if (Nr > 0) {
+          const int rr = src.fn_ri(src.user, sh, hri.p + PR * col);
+          require(rr == 0, "three_index_ri callback failed for shell " + std::to_string(sh), -6);
+        }
+        col += src.shells[sh];
+      }
+      UW12G_HIP(hipMemcpyAsync(st3.p, h3.p, sizeof(double) * P * na, hipMemcpyHostToDevice, s));
+      if (Nr > 0) UW12G_HIP(hipMemcpyAsync(stri.p, hri.p, sizeof(double) * PR * na, hipMemcpyHostToDevice, s));
       x3 = st3.p;
       xri = stri.p;
     }
@@ -1183,13 +1267,41 @@ void uw12g_context_destroy(uw12g_context* ctx) {
 
 int uw12g_setup_base_integrals(uw12g_context* ctx, const double* X3, const double* X2, const double* X3_ri,
                                size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out) {
-  return guard([&] { setup_base(ctx, X3, X2, X3_ri, n_ao, n_df, n_ri, false, out); });
+  return guard([&] {
+    ThreeIndexSource src;
+    src.X3 = X3;
+    src.X3ri = X3_ri;
+    require(X3 != nullptr, "null integral array");
+    setup_base(ctx, src, X2, n_ao, n_df, n_ri, out);
+  });
 }
 
 int uw12g_setup_base_integrals_device(uw12g_context* ctx, const double* X3_dev, const double* X2,
                                       const double* X3_ri_dev, size_t n_ao, size_t n_df, size_t n_ri,
                                       uw12g_base_integrals** out) {
-  return guard([&] { setup_base(ctx, X3_dev, X2, X3_ri_dev, n_ao, n_df, n_ri, true, out); });
+  return guard([&] {
+    ThreeIndexSource src;
+    src.X3 = X3_dev;
+    src.X3ri = X3_ri_dev;
+    src.device = true;
+    require(X3_dev != nullptr, "null integral array");
+    setup_base(ctx, src, X2, n_ao, n_df, n_ri, out);
+  });
+}
+
+int uw12g_setup_base_integrals_fn(uw12g_context* ctx, const double* X2, uw12g_three_index_fn three_index,
+                                  uw12g_three_index_fn three_index_ri, void* user, const size_t* df_sizes,
+                                  size_t n_shells, size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out) {
+  return guard([&] {
+    require(three_index != nullptr, "null three_index callback");
+    require(df_sizes != nullptr && n_shells > 0, "no density-fitting shells");
+    ThreeIndexSource src;
+    src.fn = three_index;
+    src.fn_ri = three_index_ri;
+    src.user = user;
+    src.shells.assign(df_sizes, df_sizes + n_shells);
+    setup_base(ctx, src, X2, n_ao, n_df, n_ri, out);
+  });
 }
 
 void uw12g_base_integrals_destroy(uw12g_base_integrals* ints) {
@@ -1390,6 +1502,13 @@ int uw12g_fock(uw12g_context* ctx, double* fock, const uw12g_base_integrals* W, 
   return uw12g_form_fock(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active,
                          !nearly_zero(scale_same_spin), scale_opp_spin, scale_same_spin, UW12G_FP64, fock, nullptr,
                          energy);
+}
+
+int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles) {
+  return guard([&] {
+    require(ctx != nullptr && doubles > 0, "invalid argument");
+    ctx->stream_cap = doubles;
+  });
 }
 
 int uw12g_context_set_profiling(uw12g_context* ctx, int on) {

@@ -208,3 +208,27 @@ def random_system(n_ao, n_df, n_ri, n_occ, n_active=None, seed=11, uniform_orbit
         n_active = list(n_occ)
     return dict(name='random', N=n_ao, Nd=n_df, Nr=n_ri, S=S, ints=ints, orbitals=orbitals, occ=occ,
                 n_active=list(n_active))
+
+
+def shell_integrals(system, kind):
+    """Per-DF-shell generators for one operator of a make_system() system, the
+    reference's ThreeIndexFn form (integral_functions.hpp:16-55): a shell is the
+    set of DF functions on one atom.  Returns (X2, df_sizes, fn, fn_ri) where
+    fn(shell) / fn_ri(shell) compute that shell's packed-AO / RI columns on
+    demand (fn_ri is None for WV)."""
+    eao, cao = system['basis']['ao']
+    edf, cdf = system['basis']['df']
+    eri, cri = system['basis']['ri']
+    starts = [0] + [i for i in range(1, len(edf)) if not np.array_equal(cdf[i], cdf[i - 1])] + [len(edf)]
+    sizes = [b - a for a, b in zip(starts[:-1], starts[1:])]
+
+    def fn(shell):
+        a, b = starts[shell], starts[shell + 1]
+        return three_index(kind, eao, cao, np.ascontiguousarray(edf[a:b]), np.ascontiguousarray(cdf[a:b]))
+
+    def fn_ri(shell):
+        a, b = starts[shell], starts[shell + 1]
+        return three_index_ri(kind, eri, cri, eao, cao, np.ascontiguousarray(edf[a:b]),
+                              np.ascontiguousarray(cdf[a:b]))
+
+    return two_index(kind, edf, cdf), sizes, fn, (fn_ri if kind != 'WV' else None)
