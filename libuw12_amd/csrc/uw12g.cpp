@@ -1086,6 +1086,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
         double* X2sT = ctx->buf("blk3", size_t(ao) * std::max(DW, DV));
+        // fock_k's R needs Q~[mu', i, alpha | j] = sum_B PhiT_V[mu', j, B] sV_B Phi_V[alpha, c+i, B]
+        // (indirect_utils.cpp:199-259), which is the first pass's K~[mu', (alpha, c+i) | j]: R and
+        // fock_k are formed from each K~ block of that pass before it is overwritten
+        double* R = ctx->buf("blk4", size_t(DW) * N * nj);
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
@@ -1131,6 +1135,43 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             h.Cm = lin(1);
             h.Cn = lin(int64_t(MN));
             run_gemm(ctx, h, "3e_ind_fock_l");
+            if (pass == 0) {
+              GemmDesc r;  // R[A, alpha | j] = sum_{(mu', i)} Phi_W[mu', c+i, A] K~[mu', (alpha, c+i) | j]
+              r.M = DW;
+              r.N = N;
+              r.K = M * a;
+              r.batch = jn;
+              r.A = q.phiW + size_t(M) * c;
+              r.Am = lin(int64_t(M) * o);
+              r.Ak = lin(1);
+              r.B = Kt + MN * size_t(c);
+              r.Bk = two(uint32_t(M), 1, int64_t(MN));
+              r.Bn = lin(M);
+              r.Bb = MNo;
+              r.C = R;
+              r.Cm = lin(1);
+              r.Cn = lin(DW);
+              r.Cb = int64_t(DW) * N;
+              run_gemm(ctx, r, "3e_ind_fock_r");
+              GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
+              f.M = N;
+              f.N = N;
+              f.K = DW * jn;
+              f.alpha = k1;
+              f.beta = 1.0;
+              f.A = q.phiW + size_t(M) * (c + j0);
+              f.Am = lin(1);
+              f.Ak = two(uint32_t(DW), int64_t(M) * o, M);
+              f.ks = W->sign.p;
+              f.Km = two(uint32_t(DW), 1, 0);
+              f.B = R;
+              f.Bk = two(uint32_t(DW), 1, int64_t(DW) * N);
+              f.Bn = lin(DW);
+              f.C = Fraw + NN * s;
+              f.Cm = lin(1);
+              f.Cn = lin(N);
+              run_gemm(ctx, f, "3e_ind_fock_k");
+            }
           }
           GemmDesc f;  // F += k1 sum_{mu', A} Gamma_X[mu', alpha, A] L[mu', beta, A]
           f.M = N;
@@ -1148,68 +1189,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           f.Cm = lin(1);
           f.Cn = lin(N);
           run_gemm(ctx, f, "3e_ind_fock_f");
-        }
-        // fock_k (indirect_utils.cpp:199-259), blocked over j
-        const int64_t MaN = int64_t(M) * a * N;
-        const int nj2 = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MaN, 1))));
-        double* Qt = ctx->buf("blk1", size_t(MaN) * nj2);
-        double* R = ctx->buf("blk2", size_t(DW) * N * nj2);
-        for (int j0 = 0; j0 < a; j0 += nj2) {
-          const int jn = std::min(nj2, a - j0);
-          GemmDesc g;  // Q~[mu', i, alpha | j] = sum_B PhiT_V[mu', j, B] sV_B Phi_V[alpha, c+i, B]
-          g.M = M;
-          g.N = N * a;
-          g.K = DV;
-          g.batch = jn;
-          g.A = q.phitV + size_t(M) * j0;
-          g.Am = lin(1);
-          g.Ak = lin(int64_t(M) * a);
-          g.Ab = M;
-          g.ks = V->sign.p;
-          g.Km = lin(1);
-          g.B = q.phiV + size_t(M) * c;
-          g.Bk = lin(int64_t(M) * o);
-          g.Bn = two(uint32_t(N), 1, M);
-          g.C = Qt;
-          g.Cm = lin(1);
-          g.Cn = two(uint32_t(N), int64_t(M) * a, M);
-          g.Cb = MaN;
-          run_gemm(ctx, g, "3e_ind_fock_q");
-          GemmDesc h;  // R[A, alpha | j] = sum_{(mu', i)} Phi_W[mu', c+i, A] Q~[(mu', i), alpha | j]
-          h.M = DW;
-          h.N = N;
-          h.K = M * a;
-          h.batch = jn;
-          h.A = q.phiW + size_t(M) * c;
-          h.Am = lin(int64_t(M) * o);
-          h.Ak = lin(1);
-          h.B = Qt;
-          h.Bk = lin(1);
-          h.Bn = lin(int64_t(M) * a);
-          h.Bb = MaN;
-          h.C = R;
-          h.Cm = lin(1);
-          h.Cn = lin(DW);
-          h.Cb = int64_t(DW) * N;
-          run_gemm(ctx, h, "3e_ind_fock_r");
-          GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
-          f.M = N;
-          f.N = N;
-          f.K = DW * jn;
-          f.alpha = k1;
-          f.beta = 1.0;
-          f.A = q.phiW + size_t(M) * (c + j0);
-          f.Am = lin(1);
-          f.Ak = two(uint32_t(DW), int64_t(M) * o, M);
-          f.ks = W->sign.p;
-          f.Km = two(uint32_t(DW), 1, 0);
-          f.B = R;
-          f.Bk = two(uint32_t(DW), 1, int64_t(DW) * N);
-          f.Bn = lin(DW);
-          f.C = Fraw + NN * s;
-          f.Cm = lin(1);
-          f.Cn = lin(N);
-          run_gemm(ctx, f, "3e_ind_fock_k");
         }
       }
     }
