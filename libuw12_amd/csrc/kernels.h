@@ -10,6 +10,8 @@ namespace uw12g {
 constexpr int kMaxPartials = 4096;  // scratch doubles needed by every reduction
 
 // sum_i x[i] * (y ? y[i] : x[i]) * (row_w ? row_w[i % ld] : 1) * (col_w ? col_w[(i / ld) % col_n] : 1)
+// sum_i x(i) y(i) row_w[i % ld] col_w[(i / ld) % col_n] over i < n; element i of
+// x sits at x[i % ld + (i / ld) xs] when xs > 0 (else x[i]), likewise y with ys.
 struct DotDesc {
   const double* x = nullptr;
   const double* y = nullptr;
@@ -18,6 +20,7 @@ struct DotDesc {
   const double* col_w = nullptr;
   int64_t ld = 1;
   int64_t col_n = 1;
+  int64_t xs = 0, ys = 0;
 };
 
 void assemble_gamma(const double* X3p, const double* X3ri, double* G, int N, int Nr, int na, hipStream_t s);

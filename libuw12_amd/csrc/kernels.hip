@@ -98,7 +98,13 @@ __global__ void gemv_n_kernel(const double* __restrict__ G, int64_t rows, int64_
 __global__ void dot_kernel(DotDesc d, double* partial) {
   double acc = 0.0;
   for (int64_t i = blockIdx.x * int64_t(RB) + threadIdx.x; i < d.n; i += int64_t(gridDim.x) * RB) {
-    double v = d.x[i] * (d.y ? d.y[i] : d.x[i]);
+    int64_t ix = i, iy = i;
+    if (d.xs > 0 || d.ys > 0) {
+      const int64_t col = i / d.ld, r = i - col * d.ld;
+      if (d.xs > 0) ix = r + col * d.xs;
+      if (d.ys > 0) iy = r + col * d.ys;
+    }
+    double v = d.x[ix] * (d.y ? d.y[iy] : d.x[ix]);
     if (d.row_w) v *= d.row_w[i % d.ld];
     if (d.col_w) v *= d.col_w[(i / d.ld) % d.col_n];
     acc += v;

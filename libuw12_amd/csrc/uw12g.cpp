@@ -1027,8 +1027,9 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           run_gemm(ctx, hf, "4e_ind_fock_f");
         }
       }
-      // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k
-      if (mode != kAdjoint) {
+      // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k.  With the
+      // Fock it comes from the Fock's L intermediate instead (below).
+      if (mode != kAdjoint && !calc_fock) {
         const int64_t Ma = int64_t(M) * a;
         const size_t cap = ctx->block_cap;
         int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
@@ -1172,6 +1173,37 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               f.Cn = lin(N);
               run_gemm(ctx, f, "3e_ind_fock_k");
             }
+          }
+          if (pass == 0 && mode != kAdjoint) {
+            // E3_ind = e k1 sum_{mu', i, A} Phi_W[mu', c+i, A] sum_beta L[mu', beta, A] Ca[beta, i]:
+            // L holds sum_{j,k,B} PhiT_V[mu', j, B] sV_B Phi_V[beta, k, B] W2s[(j, k), A], and Ca^T
+            // over beta turns sV_B Phi_V[beta, k, B] into V2s[(i, k), B] (indirect_utils.cpp:19-133)
+            double* Lc = ctx->buf("blk5", size_t(M) * a * DW);
+            GemmDesc g;
+            g.M = M;
+            g.N = a;
+            g.K = N;
+            g.batch = DW;
+            g.A = L;
+            g.Am = lin(1);
+            g.Ak = lin(M);
+            g.Ab = int64_t(MN);
+            g.B = q.Co + size_t(N) * c;
+            g.Bk = lin(1);
+            g.Bn = lin(N);
+            g.C = Lc;
+            g.Cm = lin(1);
+            g.Cn = lin(M);
+            g.Cb = int64_t(M) * a;
+            run_gemm(ctx, g, "3e_ind_e_lc");
+            DotDesc dd;
+            dd.x = q.phiW + size_t(M) * c;
+            dd.y = Lc;
+            dd.n = int64_t(M) * a * DW;
+            dd.ld = int64_t(M) * a;
+            dd.xs = int64_t(M) * o;
+            dd.ys = int64_t(M) * a;
+            dot(dd, e * k1, Eacc + UW12G_E3_INDIRECT, 1, partial, st);
           }
           GemmDesc f;  // F += k1 sum_{mu', A} Gamma_X[mu', alpha, A] L[mu', beta, A]
           f.M = N;

@@ -30,7 +30,13 @@ def test_adjoint_equals_fused(sys_open, prec, kappa):
     e, tape = u.forward(*args, indirect_term=kappa[1] != 0, scale_opp_spin=kappa[0], scale_same_spin=kappa[1],
                         precision=prec)
     F = u.adjoint(tape)
-    assert all(e[k] == ef[k] for k in u.TERMS) and e['total'] == ef['total']
+    # the forward forms the same-spin 3e energy from GW/GV, the fused path from its
+    # Fock intermediate L: equal to rounding; every other piece bitwise
+    for k in u.TERMS:
+        if k == 'e3_indirect':  # FP32 form: two differently rounded FP32 contraction orders
+            assert abs(e[k] - ef[k]) <= (1e-12 if prec == 0 else 1e-5) * max(1.0, abs(ef[k]))
+        else:
+            assert e[k] == ef[k], k
     for a, b in zip(F, Ff):
         assert np.abs(a - b).max() <= 1e-14 * np.abs(b).max()
     F2 = u.adjoint(tape)  # the adjoint leaves the tape valid

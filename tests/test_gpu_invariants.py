@@ -61,7 +61,15 @@ def test_energy_only_and_indirect_flag(rsys):
     s, h = rsys
     e1, F1 = gpu_eval(h, s, 1.0, 0.5)
     e2, F2 = gpu_eval(h, s, 1.0, 0.5, fock=False)
-    assert F2 is None and e1['total'] == e2['total']
+    assert F2 is None
+    # with the Fock the same-spin 3e energy is read off the Fock's L intermediate,
+    # without it from the GW/GV blocks: equal up to rounding, every other piece bitwise
+    for k in TERMS:
+        if k == 'e3_indirect':
+            assert abs(e1[k] - e2[k]) <= 1e-12 * abs(e2[k])
+        else:
+            assert e1[k] == e2[k], k
+    assert abs(e1['total'] - e2['total']) <= 1e-12 * abs(e2['total'])
     a, Fa = gpu_eval(h, s, 1.0, 0.0, indirect=True)
     b, Fb = gpu_eval(h, s, 1.0, 0.0, indirect=False)
     assert a['total'] == b['total']
