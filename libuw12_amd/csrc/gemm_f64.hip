@@ -401,10 +401,12 @@ __global__ void gemm_naive_kernel(GemmDesc d) {
   }
 }
 
+// Profiling name, matching the template arguments of the instantiation
+// (gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>).
 template <int BM, int BN>
 std::string cfg_name(bool akc, bool bkc, bool ks, int prec) {
-  return std::string(prec ? "gemm_f32_kernel<" : "gemm_f64_kernel<") + std::to_string(BM) + "," + std::to_string(BN) +
-         "," + (akc ? "1" : "0") + "," + (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + ">";
+  return "gemm_f64_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
+         (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + "," + std::to_string(prec) + ">";
 }
 
 template <int BM, int BN, bool KS, int PREC>
