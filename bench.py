@@ -169,8 +169,11 @@ def main():
     import libuw12_amd as u
     from libuw12_amd import synth
 
+    # the larger configurations stream their integrals per DF shell (ThreeIndexFn form)
+    # instead of holding X3 / X3_ri on the host
+    stream = args.config in ('c3', 'c4')
     t = time.perf_counter()
-    system = synth.make_system(args.config)
+    system = synth.make_system(args.config, with_integrals=not stream)
     t_gen = time.perf_counter() - t
     N, Nd, Nr = system['N'], system['Nd'], system['Nr']
     occ_counts = [len(x) for x in system['occ']]
@@ -178,9 +181,17 @@ def main():
           file=sys.stderr, flush=True)
     ctx = u.Context(0)
     t = time.perf_counter()
-    W = u.setup_base_integrals(*system['ints']['W'], n_ao=N, context=ctx)
-    V = u.setup_base_integrals(*system['ints']['V'], n_ao=N, context=ctx)
-    WV = u.setup_base_integrals(system['ints']['WV'][0], system['ints']['WV'][1], None, n_ao=N, context=ctx)
+    if stream:
+        handles = []
+        for kind in ('W', 'V', 'WV'):
+            X2, sizes, fn, fn_ri = synth.shell_integrals(system, kind)
+            handles.append(u.setup_base_integrals_fn(X2, fn, fn_ri, sizes, N, Nr if fn_ri else 0, context=ctx))
+            print(f'[bench] {kind} integrals streamed ({len(sizes)} DF shells)', file=sys.stderr, flush=True)
+        W, V, WV = handles
+    else:
+        W = u.setup_base_integrals(*system['ints']['W'], n_ao=N, context=ctx)
+        V = u.setup_base_integrals(*system['ints']['V'], n_ao=N, context=ctx)
+        WV = u.setup_base_integrals(system['ints']['WV'][0], system['ints']['WV'][1], None, n_ao=N, context=ctx)
     P = u.setup_abs_projectors(system['S'], N, Nr, context=ctx)
     t_setup = time.perf_counter() - t
     print(f'[bench] device setup {t_setup:.1f}s (kept DF directions W={W.n_kept} V={V.n_kept} WV={WV.n_kept})',
@@ -236,7 +247,7 @@ def main():
     print(f'[bench] FP32 form: {1e3 * t32:.1f} ms, total energy error {fp32["term_error_eh"]["total"]:.3e} Eh',
           file=sys.stderr, flush=True)
     cpu = None
-    if args.cpu_baseline == 'auto':
+    if args.cpu_baseline == 'auto' and not stream:
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
         cpu = cpu_baseline(system, k0, k1)
     out = {
