@@ -11,6 +11,8 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <map>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -428,6 +430,65 @@ void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
   }
 }
 
+template <int BM, int BN, bool KS, int PREC>
+const void* kernel_ptr_ks(bool akc, bool bkc) {
+  if (akc && bkc) return reinterpret_cast<const void*>(&gemm_f64_kernel<BM, BN, true, true, KS, PREC>);
+  if (akc) return reinterpret_cast<const void*>(&gemm_f64_kernel<BM, BN, true, false, KS, PREC>);
+  if (bkc) return reinterpret_cast<const void*>(&gemm_f64_kernel<BM, BN, false, true, KS, PREC>);
+  return reinterpret_cast<const void*>(&gemm_f64_kernel<BM, BN, false, false, KS, PREC>);
+}
+
+// Resident workgroups per CU of one instantiation (cached) times the CU count.
+template <int BM, int BN>
+int64_t gemm_slots(bool akc, bool bkc, bool ks, int prec) {
+  static std::mutex mu;
+  static std::map<int, int64_t> cache;
+  const int key = (akc ? 1 : 0) | (bkc ? 2 : 0) | (ks ? 4 : 0) | (prec << 3);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const void* f = prec ? (ks ? kernel_ptr_ks<BM, BN, true, 1>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 1>(akc, bkc))
+                       : (ks ? kernel_ptr_ks<BM, BN, true, 0>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 0>(akc, bkc));
+  int per_cu = 0, dev = 0, cus = 0;
+  UW12G_HIP(hipGetDevice(&dev));
+  UW12G_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  UW12G_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, 0));
+  const int64_t slots = int64_t(std::max(per_cu, 1)) * std::max(cus, 1);
+  cache[key] = slots;
+  return slots;
+}
+
+int64_t slots_for(int BM, int BN, bool akc, bool bkc, bool ks, int prec) {
+  if (BM == 128 && BN == 128) return gemm_slots<128, 128>(akc, bkc, ks, prec);
+  if (BM == 128) return gemm_slots<128, 64>(akc, bkc, ks, prec);
+  if (BN == 128) return gemm_slots<64, 128>(akc, bkc, ks, prec);
+  return gemm_slots<64, 64>(akc, bkc, ks, prec);
+}
+
+// Split of the k range: the number of chunks that maximises the useful
+// fraction of the workgroup waves (a grid of w workgroups on S resident slots
+// runs ceil(w / S) waves), discounted by the partial-sum reduction, whose cost
+// relative to the GEMM is about 60 (2 s + 1) / K (8 bytes per partial element
+// moved at ~1/60 of the FP64 matrix rate).
+int choose_splits(int64_t wgs, int kt, int K, int batch, int64_t slots) {
+  auto useful = [&](int s) {
+    const int64_t units = wgs * s;
+    const int64_t waves = (units + slots - 1) / slots;
+    return double(units) / double(waves * slots);
+  };
+  int best_s = 1;
+  double best = useful(1);
+  for (int s = 2; s <= 64; ++s) {
+    if (kt / s < 8 || int64_t(s) * batch > 65535) break;
+    const double e = useful(s) / (1.0 + 60.0 * (2 * s + 1) / double(std::max(K, 1)));
+    if (e > best + 0.01) {
+      best = e;
+      best_s = s;
+    }
+  }
+  return best_s;
+}
+
 // Unit stride on the low level of a map (contiguous runs along that index).
 bool unit(const Map2& m) { return m.s0 == 1 && m.n0 >= 2; }
 
@@ -504,11 +565,8 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   int splits = 1;
   const int64_t wgs = tiles * d.batch;
   const int kt = (klen + BK - 1) / BK;
-  if (kbase == 0 && wgs < 512 && kt >= 16 && ws) {
-    splits = int(std::min<int64_t>((1024 + wgs - 1) / wgs, kt / 8));
-    splits = std::max(1, std::min(splits, 64));
-    while (splits > 1 && int64_t(splits) * d.batch > 65535) --splits;
-  }
+  if (kbase == 0 && kt >= 16 && ws)
+    splits = choose_splits(wgs, kt, klen, d.batch, slots_for(BM, BN, akc, bkc, d.ks != nullptr, d.prec));
   if (d.prec == 1 && klen > kF32Chunk) {
     // FP32 accumulation runs over at most kF32Chunk k values; chunks add up in FP64
     if (kbase != 0 || !ws) throw Error(-2, "gemm: FP32 k remainder longer than one chunk");

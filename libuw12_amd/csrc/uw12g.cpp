@@ -960,6 +960,11 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         gx.Cn = lin(N);
         gx.Cb = int64_t(N) * ao;
         run_gemm(ctx, gx, "4e_ind_fock_x");
+        // V2sT[B + DV (i + a k)] = sV_B O2_V[i + a k, B]: the DF index contiguous, so the Y_i
+        // contraction reads its B operand along unit stride
+        double* V2sT = ctx->buf("V2sT", size_t(ao) * DV);
+        timed(ctx, "4e_ind_fock_v2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DV,
+              [&] { transpose_scale(q.o2V, ao, DV, V->sign.p, V2sT, st); });
         GemmDesc gy;  // Y_i[(beta, j), k] = sum_B Phi_V[beta, c+j, B] sV_B O2_V[i + a k, B]
         gy.M = N * a;
         gy.N = o;
@@ -968,10 +973,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         gy.A = q.phiV + size_t(M) * c;
         gy.Am = two(uint32_t(N), 1, M);
         gy.Ak = lin(int64_t(M) * o);
-        gy.B = V2s;
-        gy.Bk = lin(ao);
-        gy.Bn = lin(a);
-        gy.Bb = 1;
+        gy.B = V2sT;
+        gy.Bk = lin(1);
+        gy.Bn = lin(int64_t(DV) * a);
+        gy.Bb = DV;
         gy.C = Yb;
         gy.Cm = lin(1);
         gy.Cn = lin(int64_t(N) * a);
