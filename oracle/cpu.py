@@ -42,6 +42,8 @@ def lib():
                                          ctypes.c_double, ctypes.c_int, dp, dp]
         _lib.oracle_set_pair_stride.argtypes = [ctypes.c_int]
         _lib.oracle_set_pair_stride.restype = None
+        _lib.oracle_set_progress.argtypes = [ctypes.c_int]
+        _lib.oracle_set_progress.restype = None
         _lib.oracle_abs_projector.argtypes = [dp, ctypes.c_int, ctypes.c_int, dp]
         _lib.oracle_eig_sym.argtypes = [ctypes.c_int, dp, dp, dp]
         _lib.oracle_blas_kind.restype = ctypes.c_int
@@ -153,6 +155,11 @@ class System:
 def set_pair_stride(stride):
     """Evaluate only every stride-th outer same-spin pair index (CPU-baseline sampling)."""
     lib().oracle_set_pair_stride(int(stride))
+
+
+def set_progress(on):
+    """Print completed same-spin pair-loop iterations to stderr (long evaluations)."""
+    lib().oracle_set_progress(1 if on else 0)
 
 
 def abs_projector(S, n_ao, n_ri):

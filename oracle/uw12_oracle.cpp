@@ -35,6 +35,7 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -59,6 +60,23 @@ int g_force_builtin = 0;
 // evaluate a uniform sample of the pair loops; the CPU-baseline timing uses it
 // to bound its run time and extrapolates.
 int g_stride = 1;
+int g_progress = 0;  // print completed pair-loop iterations to stderr (long runs)
+
+// Completion count of one parallel pair loop, printed every ~5% when enabled.
+struct Progress {
+  const char* what;
+  long total, step;
+  std::atomic<long> done{0};
+  Progress(const char* w, long t) : what(w), total(t), step(std::max(1L, t / 20)) {}
+  void tick() {
+    if (!g_progress) return;
+    const long d = ++done;
+    if (d % step == 0 || d == total) {
+      std::fprintf(stderr, "[oracle] %s %ld/%ld\n", what, d, total);
+      std::fflush(stderr);
+    }
+  }
+};
 
 void blas_init() {
   if (g_blas_init) return;
@@ -622,7 +640,7 @@ Mat xab_dttilde(const Mat& W1, const Mat& V1, const Mat& xab, const std::vector<
 
 // calculate_ttilde_dxab_incore: s term (direct_utils.cpp:256-286) + p term (:302-334)
 Mat ttilde_dxab(const Base& Wb, const Base& Vb, const Mat& tt, const ABS& P) {
-  const int N = Wb.N, Nr = Wb.Nr, PP = N * (N + 1) / 2, NdW = Wb.Nd;
+  const int N = Wb.N, Nr = Wb.Nr, NdW = Wb.Nd;
   Mat s = mul(Vb.J3, 'N', tt, 'T');    // (PP x NdW)
   Mat p = mul(Vb.J3ri, 'N', tt, 'T');  // (N Nr x NdW)
   const int nth = omp_get_max_threads();
@@ -685,6 +703,7 @@ double indirect_3el_energy(const Ints& W, const Ints& V, const ABS& P, int s) {
   const Mat &W1 = W.one[s], &V1 = V.one[s], &Wr = W.one_ri[s], &Vr = V.one_ri[s];
   const Mat &W2 = W.two[s], &V2 = V.two[s];
   double acc = 0;
+  Progress prog("indirect 3e energy pairs", long((a + g_stride - 1) / g_stride) * a);
 #pragma omp parallel reduction(+ : acc)
   {
     Mat x(NdW, NdV), t1(Nr, NdV), t2(N, NdV), Wt(o, NdW), Vt(o, NdV), ttij(NdW, NdV);
@@ -707,6 +726,7 @@ double indirect_3el_energy(const Ints& W, const Ints& V, const ABS& P, int s) {
           for (int k = 0; k < o; ++k) Vt(k, B) = V2(i + a * k, B) * V.b->vals[B];
         gemm('T', 'N', NdW, NdV, o, 1.0, Wt.d.data(), o, Vt.d.data(), o, 0.0, ttij.d.data(), NdW);
         acc += (ns == 1 ? 2 : 1) * dot(x.d.data(), ttij.d.data(), x.d.size());
+        prog.tick();
       }
   }
   return acc;
@@ -720,6 +740,7 @@ Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
   const Mat &X2 = X.two[s], &Y1 = Y.one[s], &Yr = Y.one_ri[s];
   const int nth = omp_get_max_threads();
   std::vector<Mat> part(nth, Mat(N, N));
+  Progress prog("indirect 3e Fock (i, j terms) outer k", long((o + g_stride - 1) / g_stride));
 #pragma omp parallel
   {
     Mat& f = part[omp_get_thread_num()];
@@ -747,6 +768,7 @@ Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
         gemm('N', 'N', N, N, N, 1.0, P.aa.d.data(), N, ypj.d.data(), N, 0.0, t2.d.data(), N);
         gemm('N', 'N', N, N, N, 1.0, sq.d.data(), N, t2.d.data(), N, 1.0, f.d.data(), N);
       }
+      prog.tick();
     }
   }
   Mat F(N, N);
@@ -761,6 +783,7 @@ Mat fock_k_sum(const Ints& W, const Ints& V, const ABS& P, int s) {
   const Mat &W1 = W.one[s], &V1 = V.one[s], &Wr = W.one_ri[s], &Vr = V.one_ri[s];
   const int nth = omp_get_max_threads();
   std::vector<Mat> part(nth, Mat(N, N));
+  Progress prog("indirect 3e Fock (k term) pairs", long((a + g_stride - 1) / g_stride) * a);
 #pragma omp parallel
   {
     Mat& f = part[omp_get_thread_num()];
@@ -784,6 +807,7 @@ Mat fock_k_sum(const Ints& W, const Ints& V, const ABS& P, int s) {
         gemm('T', 'N', N, N, N, 1.0, Wjb_ip.d.data(), N, t2.d.data(), N, 1.0, f.d.data(), N);
         gemm('N', 'N', N, N, N, 1.0, P.aa.d.data(), N, Vpj_ai.d.data(), N, 0.0, t2.d.data(), N);
         gemm('T', 'N', N, N, N, 1.0, Wjb_ip.d.data(), N, t2.d.data(), N, 1.0, f.d.data(), N);
+        prog.tick();
       }
   }
   Mat F(N, N);
@@ -792,7 +816,7 @@ Mat fock_k_sum(const Ints& W, const Ints& V, const ABS& P, int s) {
 }
 
 void three_indirect(const Ints& W, const Ints& V, const ABS& P, bool calc_fock, double k1, Result& R) {
-  const int ns = (int)W.Co.size(), N = W.b->N;
+  const int ns = (int)W.Co.size();
   double E = 0;
   for (int s = 0; s < ns; ++s) E += indirect_3el_energy(W, V, P, s);
   R.e3i += k1 * E;
@@ -930,6 +954,7 @@ int oracle_uw12(const double* W3, const double* W2, const double* W3ri, const do
 }
 
 void oracle_set_pair_stride(int stride) { g_stride = stride < 1 ? 1 : stride; }
+void oracle_set_progress(int on) { g_progress = on; }
 
 // The ABS+ projector blocks as one (n_ao+n_ri)^2 matrix, AO first.
 int oracle_abs_projector(const double* S, int n_ao, int n_ri, double* out) {
