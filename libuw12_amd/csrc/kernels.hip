@@ -195,6 +195,19 @@ __global__ void scale_cols_kernel(const double* __restrict__ in, int64_t rows, i
     out[e] = in[e] * sc[e / rows];
 }
 
+// One block per output column (i, b), the threads along the rows: reads and
+// writes are both contiguous runs of `rows` doubles.
+__global__ __launch_bounds__(256) void scale_rows_kernel(const double* __restrict__ in, int64_t rows, int64_t ni,
+                                                         int64_t ncol, int64_t si, int64_t sb,
+                                                         const double* __restrict__ d, double* __restrict__ out) {
+  for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
+    const int64_t i = col % ni, b = col / ni;
+    const double* src = in + si * i + sb * b;
+    double* dst = out + rows * col;
+    for (int64_t m = threadIdx.x; m < rows; m += blockDim.x) dst[m] = d[m] * src[m];
+  }
+}
+
 // out[c + cols r] = in[r + rows c] sc[c]: 64 x 64 tiles through LDS, so both the
 // reads (along r) and the writes (along c) are coalesced.
 __global__ __launch_bounds__(256) void transpose_scale_kernel(const double* __restrict__ in, int64_t rows,
@@ -309,6 +322,15 @@ void transpose_scale(const double* in, int64_t rows, int64_t cols, const double*
 void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s) {
   if (rows <= 0 || cols <= 0) return;
   hipLaunchKernelGGL(scale_cols_kernel, dim3(grid_for(rows * cols)), dim3(256), 0, s, in, rows, cols, sc, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t si, int64_t sb, const double* d,
+                double* out, hipStream_t s) {
+  const int64_t ncol = ni * nb;
+  if (rows <= 0 || ncol <= 0) return;
+  const unsigned grid = unsigned(std::min<int64_t>(ncol, 65536));
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(grid), dim3(256), 0, s, in, rows, ni, ncol, si, sb, d, out);
   UW12G_HIP(hipGetLastError());
 }
 

@@ -19,6 +19,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -217,12 +218,21 @@ struct uw12g_base_integrals {
   DBuf gamma;  // M N D
   DBuf sign;   // D
   std::vector<double> sign_host;
+  // Gamma rotated onto the kept eigenvectors of one ABS overlap: U^T Gamma
+  // (r N D, built on the first evaluation that pairs these integrals with the
+  // projector whose id is gt_id, replaced when another projector comes)
+  mutable DBuf gt;
+  mutable uint64_t gt_id = 0;
 };
 
 struct uw12g_abs_projectors {
   uw12g_context* ctx = nullptr;
   int N = 0, Nr = 0, M = 0;
-  DBuf Sp;  // M x M
+  int r = 0;       // kept eigenvectors of S
+  uint64_t id = 0;  // distinct for every projector created in the process
+  DBuf Sp;    // M x M, S+ = U diag(dinv) U^T
+  DBuf U;     // M x r kept eigenvectors
+  DBuf dinv;  // r: 1 / eigenvalue
 };
 
 namespace {
@@ -491,10 +501,12 @@ struct SpinData {
   double* Co = nullptr;  // N x o (device)
   double* wk = nullptr;  // o   (1 + [k active]) when a > 0 else 1
   double* phiW = nullptr;
-  double* phiV = nullptr;  // M o D
+  double* phiV = nullptr;  // N o D: AO rows of Gamma Co
+  double* psiW = nullptr;
+  double* psiV = nullptr;  // Mr o D: (U^T Gamma) Co, the ABS rows rotated onto S's kept eigenvectors
   double* o2W = nullptr;
   double* o2V = nullptr;  // a o D
-  double* phitV = nullptr;  // M a D_V
+  double* phitV = nullptr;  // Mr a D_V: diag(1/s) Psi_V[:, act]
   double* t = nullptr;      // D_W x D_V
   double* x = nullptr;      // D_W x D_V
 };
@@ -502,6 +514,42 @@ struct SpinData {
 double spin_factor(int ns, int s, int t, double k0, double k1) {  // four_electron_utils.cpp:9-30
   if (ns == 1) return 2.0 * (k0 + k1);
   return s == t ? k1 : k0;
+}
+
+// U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
+// eigenvectors U of the ABS overlap.  With S+ = U diag(1/s) U^T every
+// contraction over the ABS index runs over those r rows, S+ becoming the
+// diagonal 1/s (ri_utils.hpp:45-74 drops the other M - r directions).  Built
+// in FP64 once per (integrals, projector) pair and kept on the integrals.
+void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P) {
+  if (X->gt_id == P->id) return;
+  const int M = P->M, N = X->N, Mr = P->r, D = X->D;
+  X->gt_id = 0;
+  if (X->gt.p && size_t(Mr) * N * D > X->gt.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
+  X->gt.get(size_t(std::max(Mr, 1)) * N * std::max(D, 1));
+  if (Mr > 0 && D > 0) {
+    const int prec = ctx->prec;
+    ctx->prec = 0;
+    GemmDesc g;  // Gt[m, alpha, A] = sum_mu' U[mu', m] Gamma[mu', alpha, A]
+    g.M = Mr;
+    g.N = N;
+    g.K = M;
+    g.batch = D;
+    g.A = P->U.p;
+    g.Am = lin(M);
+    g.Ak = lin(1);
+    g.B = X->gamma.p;
+    g.Bk = lin(1);
+    g.Bn = lin(M);
+    g.Bb = int64_t(M) * N;
+    g.C = X->gt.p;
+    g.Cm = lin(1);
+    g.Cn = lin(Mr);
+    g.Cb = int64_t(Mr) * N;
+    run_gemm(ctx, g, "abs_rotate");
+    ctx->prec = prec;
+  }
+  X->gt_id = P->id;
 }
 
 // What one evaluate() call does: the fused energy (+ optional Fock) pass, the
@@ -526,11 +574,13 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
   require(W->ctx == ctx && V->ctx == ctx && WV->ctx == ctx && P->ctx == ctx, "objects belong to another context");
   set_device(ctx);
   hipStream_t st = ctx->stream;
-  const int M = P->M, DW = W->D, DV = V->D, DWV = WV->D;
+  const int M = P->M, Mr = P->r, DW = W->D, DV = V->D, DWV = WV->D;
   const bool calc_fock = fock != nullptr;
   const bool do_ind = indirect_term && k1 != 0.0;
   const double e = ns == 1 ? 2.0 : 1.0;
   ctx->prof.clear();
+  ensure_rotated(ctx, W, P);
+  ensure_rotated(ctx, V, P);
 
   // ---- host: occupation-weighted orbitals and active block (utils.hpp:235-310)
   std::vector<SpinData> sd(ns);
@@ -696,21 +746,24 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     SpinData& q = sd[s];
     const int o = q.o, a = q.a, c = q.c;
     const std::string sfx = std::to_string(s);
-    q.phiW = ctx->buf("phiW" + sfx, size_t(M) * o * DW);
-    q.phiV = ctx->buf("phiV" + sfx, size_t(M) * o * DV);
+    q.phiW = ctx->buf("phiW" + sfx, size_t(N) * o * DW);
+    q.phiV = ctx->buf("phiV" + sfx, size_t(N) * o * DV);
+    q.psiW = ctx->buf("psiW" + sfx, size_t(Mr) * o * DW);
+    q.psiV = ctx->buf("psiV" + sfx, size_t(Mr) * o * DV);
     q.o2W = ctx->buf("o2W" + sfx, size_t(a) * o * DW);
     q.o2V = ctx->buf("o2V" + sfx, size_t(a) * o * DV);
     q.t = ctx->buf("t" + sfx, size_t(DW) * DV);
     q.x = ctx->buf("x" + sfx, size_t(DW) * DV);
-    q.phitV = ctx->buf("phitV" + sfx, size_t(M) * a * DV);
+    q.phitV = ctx->buf("phitV" + sfx, size_t(Mr) * a * DV);
     if (mode == kAdjoint) continue;  // kept from the forward call
     const double* Ca = q.Co + size_t(N) * c;
     for (int op = 0; op < 2; ++op) {
       const uw12g_base_integrals* X = op == 0 ? W : V;
       double* phi = op == 0 ? q.phiW : q.phiV;
+      double* psi = op == 0 ? q.psiW : q.psiV;
       double* o2 = op == 0 ? q.o2W : q.o2V;
-      GemmDesc g;  // Phi[mu', k, A] = sum_alpha Gamma[mu', alpha, A] Co[alpha, k]
-      g.M = M;
+      GemmDesc g;  // Phi[mu, k, A] = sum_alpha Gamma[mu, alpha, A] Co[alpha, k] (AO rows mu)
+      g.M = N;
       g.N = o;
       g.K = N;
       g.batch = X->D;
@@ -723,9 +776,18 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       g.Bn = lin(N);
       g.C = phi;
       g.Cm = lin(1);
-      g.Cn = lin(M);
-      g.Cb = int64_t(M) * o;
+      g.Cn = lin(N);
+      g.Cb = int64_t(N) * o;
       run_gemm(ctx, g, "phi_transform");
+      GemmDesc gp = g;  // Psi[m, k, A] = sum_alpha (U^T Gamma)[m, alpha, A] Co[alpha, k]
+      gp.M = Mr;
+      gp.A = X->gt.p;
+      gp.Ak = lin(Mr);
+      gp.Ab = int64_t(Mr) * N;
+      gp.C = psi;
+      gp.Cn = lin(Mr);
+      gp.Cb = int64_t(Mr) * o;
+      run_gemm(ctx, gp, "psi_transform");
       GemmDesc h;  // O2[i, (k, A)] = sum_mu Ca[mu, i] Phi[mu, (k, A)]
       h.M = a;
       h.N = o * X->D;
@@ -735,7 +797,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       h.Ak = lin(1);
       h.B = phi;
       h.Bk = lin(1);
-      h.Bn = lin(M);
+      h.Bn = lin(N);
       h.C = o2;
       h.Cm = lin(1);
       h.Cn = lin(a);
@@ -755,30 +817,20 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     t.Cm = lin(1);
     t.Cn = lin(DW);
     run_gemm(ctx, t, "4e_tab");
-    GemmDesc p;  // PhiT_V[mu', (i, B)] = sum_nu' S+[mu', nu'] Phi_V[nu', c+i, B]
-    p.M = M;
-    p.N = a * DV;
-    p.K = M;
-    p.A = P->Sp.p;
-    p.Am = lin(1);
-    p.Ak = lin(M);
-    p.B = q.phiV + size_t(M) * c;
-    p.Bk = lin(1);
-    p.Bn = two(uint32_t(std::max(a, 1)), M, int64_t(M) * o);
-    p.C = q.phitV;
-    p.Cm = lin(1);
-    p.Cn = lin(M);
-    run_gemm(ctx, p, "3e_abs_apply");
-    GemmDesc x;  // X[A, B] = sum_{mu', i} Phi_W[mu', c+i, A] PhiT_V[mu', i, B]  (direct_utils.cpp:62-133)
+    // S+ Phi_V[:, c+i, B] = U PhiT_V[:, i, B] with PhiT_V[m, i, B] = (1/s_m) Psi_V[m, c+i, B]
+    timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DV, 16.0 * Mr * a * DV, [&] {
+      scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, q.phitV, st);
+    });
+    GemmDesc x;  // X[A, B] = sum_{m, i} Psi_W[m, c+i, A] PhiT_V[m, i, B]  (direct_utils.cpp:62-133)
     x.M = DW;
     x.N = DV;
-    x.K = M * a;
-    x.A = q.phiW + size_t(M) * c;
-    x.Am = lin(int64_t(M) * o);
+    x.K = Mr * a;
+    x.A = q.psiW + size_t(Mr) * c;
+    x.Am = lin(int64_t(Mr) * o);
     x.Ak = lin(1);
     x.B = q.phitV;
     x.Bk = lin(1);
-    x.Bn = lin(int64_t(M) * a);
+    x.Bn = lin(int64_t(Mr) * a);
     x.C = q.x;
     x.Cm = lin(1);
     x.Cn = lin(DW);
@@ -821,8 +873,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       g.N = DV;
       g.K = DW;
       g.A = q.phiW;
-      g.Am = two(uint32_t(N), 1, M);
-      g.Ak = lin(int64_t(M) * o);
+      g.Am = lin(1);
+      g.Ak = lin(int64_t(N) * o);
       g.B = Z;
       g.Bk = lin(1);
       g.Bn = lin(DW);
@@ -837,7 +889,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       f.beta = 1.0;
       f.A = q.phiV;
       f.Am = lin(1);
-      f.Ak = lin(M);
+      f.Ak = lin(N);
       f.ks = q.wk;
       f.Km = two(uint32_t(std::max(o, 1)), 1, 0);
       f.B = G;
@@ -848,52 +900,40 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       f.Cn = lin(N);
       run_gemm(ctx, f, "direct_fock_F");
       // t~ dX/dD (direct_utils.cpp:242-347): F -= sum_AB Tb_AB Gamma_W[:, a, A]^T S+ Gamma_V[:, b, B]
-      const size_t MN = size_t(M) * N;
-      const int nb = int(std::max<size_t>(1, std::min<size_t>(DW, (size_t(1) << 27) / std::max<size_t>(MN, 1))));
-      double* Xi = ctx->buf("ttdx_xi", MN * nb);
-      double* Ps = ctx->buf("ttdx_psi", MN * nb);
-      for (int A0 = 0; A0 < DW; A0 += nb) {
+      // = sum_{m, A} Gt_W[m, a, A] (1/s_m) Xi[m, b, A], Xi = Gt_V Tb^T over B
+      const size_t MrN = size_t(Mr) * N;
+      const int nb = int(std::max<size_t>(1, std::min<size_t>(DW, (size_t(1) << 27) / std::max<size_t>(MrN, 1))));
+      double* Xi = ctx->buf("ttdx_xi", MrN * nb);
+      for (int A0 = 0; Mr > 0 && A0 < DW; A0 += nb) {
         const int na = std::min(nb, DW - A0);
-        GemmDesc a1;  // Xi[(nu', beta), A] = sum_B Gamma_V[(nu', beta), B] Tb[A0 + A, B]
-        a1.M = int(MN);
+        GemmDesc a1;  // Xi[(m, beta), A] = sum_B Gt_V[(m, beta), B] Tb[A0 + A, B]
+        a1.M = int(MrN);
         a1.N = na;
         a1.K = DV;
-        a1.A = V->gamma.p;
+        a1.A = V->gt.p;
         a1.Am = lin(1);
-        a1.Ak = lin(int64_t(MN));
+        a1.Ak = lin(int64_t(MrN));
         a1.B = Tb + A0;
         a1.Bk = lin(DW);
         a1.Bn = lin(1);
         a1.C = Xi;
         a1.Cm = lin(1);
-        a1.Cn = lin(int64_t(MN));
+        a1.Cn = lin(int64_t(MrN));
         run_gemm(ctx, a1, "3e_ttdx_xi");
-        GemmDesc a2;  // Psi = S+ Xi
-        a2.M = M;
-        a2.N = N * na;
-        a2.K = M;
-        a2.A = P->Sp.p;
-        a2.Am = lin(1);
-        a2.Ak = lin(M);
-        a2.B = Xi;
-        a2.Bk = lin(1);
-        a2.Bn = lin(M);
-        a2.C = Ps;
-        a2.Cm = lin(1);
-        a2.Cn = lin(M);
-        run_gemm(ctx, a2, "3e_ttdx_abs");
-        GemmDesc a3;  // F -= sum_{mu', A} Gamma_W[mu', alpha, A0+A] Psi[mu', beta, A]
+        GemmDesc a3;  // F -= sum_{m, A} Gt_W[m, alpha, A0+A] (1/s_m) Xi[m, beta, A]
         a3.M = N;
         a3.N = N;
-        a3.K = M * na;
+        a3.K = Mr * na;
         a3.alpha = -1.0;
         a3.beta = 1.0;
-        a3.A = W->gamma.p + MN * A0;
-        a3.Am = lin(M);
-        a3.Ak = two(uint32_t(M), 1, int64_t(MN));
-        a3.B = Ps;
-        a3.Bk = two(uint32_t(M), 1, int64_t(MN));
-        a3.Bn = lin(M);
+        a3.A = W->gt.p + MrN * A0;
+        a3.Am = lin(Mr);
+        a3.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
+        a3.ks = P->dinv.p;
+        a3.Km = two(uint32_t(Mr), 1, 0);
+        a3.B = Xi;
+        a3.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
+        a3.Bn = lin(Mr);
         a3.C = Fraw + NN * s;
         a3.Cm = lin(1);
         a3.Cn = lin(N);
@@ -904,7 +944,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
 
   // ================= same-spin (indirect) terms, x k1 =================
   if (do_ind) {
-    const size_t MN = size_t(M) * N;
     for (int s = 0; s < ns; ++s) {
       SpinData& q = sd[s];
       const int o = q.o, a = q.a, c = q.c;
@@ -948,10 +987,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         gx.N = int(ao);
         gx.K = DW;
         gx.batch = a;
-        gx.A = q.phiW + size_t(M) * c;
+        gx.A = q.phiW + size_t(N) * c;
         gx.Am = lin(1);
-        gx.Ak = lin(int64_t(M) * o);
-        gx.Ab = M;
+        gx.Ak = lin(int64_t(N) * o);
+        gx.Ab = N;
         gx.B = W2s;
         gx.Bk = lin(ao);
         gx.Bn = lin(1);
@@ -970,9 +1009,9 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         gy.N = o;
         gy.K = DV;
         gy.batch = a;
-        gy.A = q.phiV + size_t(M) * c;
-        gy.Am = two(uint32_t(N), 1, M);
-        gy.Ak = lin(int64_t(M) * o);
+        gy.A = q.phiV + size_t(N) * c;
+        gy.Am = lin(1);
+        gy.Ak = lin(int64_t(N) * o);
         gy.B = V2sT;
         gy.Bk = lin(1);
         gy.Bn = lin(int64_t(DV) * a);
@@ -1014,8 +1053,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           hy.K = DV;
           hy.batch = o;
           hy.A = q.phiV;
-          hy.Am = two(uint32_t(N), 1, M);
-          hy.Ak = lin(int64_t(M) * o);
+          hy.Am = lin(1);
+          hy.Ak = lin(int64_t(N) * o);
           hy.B = V2s;
           hy.Bk = lin(ao);
           hy.Bn = lin(1);
@@ -1035,20 +1074,20 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k.  With the
       // Fock it comes from the Fock's L intermediate instead (below).
       if (mode != kAdjoint && !calc_fock) {
-        const int64_t Ma = int64_t(M) * a;
+        const int64_t Ma = int64_t(Mr) * a;
         const size_t cap = ctx->block_cap;
         int nk = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(cap) / std::max<int64_t>(Ma * a, 1))));
         double* GWb = ctx->buf("blk1", size_t(Ma) * a * nk);
         double* GVb = ctx->buf("blk2", size_t(Ma) * a * nk);
-        for (int k0 = 0; k0 < o; k0 += nk) {
+        for (int k0 = 0; Mr > 0 && k0 < o; k0 += nk) {
           const int kn = std::min(nk, o - k0);
-          GemmDesc g;  // GW[(mu', i), (j, k)] = sum_A Phi_W[mu', c+i, A] W2s[(j, k), A]
+          GemmDesc g;  // GW[(m, i), (j, k)] = sum_A Psi_W[m, c+i, A] W2s[(j, k), A]
           g.M = int(Ma);
           g.N = a * kn;
           g.K = DW;
-          g.A = q.phiW + size_t(M) * c;
+          g.A = q.psiW + size_t(Mr) * c;
           g.Am = lin(1);
-          g.Ak = lin(int64_t(M) * o);
+          g.Ak = lin(int64_t(Mr) * o);
           g.B = W2s + int64_t(a) * k0;
           g.Bk = lin(ao);
           g.Bn = lin(1);
@@ -1056,7 +1095,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           g.Cm = lin(1);
           g.Cn = lin(Ma);
           run_gemm(ctx, g, "3e_ind_e_gw");
-          GemmDesc h = g;  // GV[(mu', j), (i, k)] = sum_B PhiT_V[mu', j, B] V2s[(i, k), B]
+          GemmDesc h = g;  // GV[(m, j), (i, k)] = sum_B PhiT_V[m, j, B] V2s[(i, k), B]
           h.K = DV;
           h.A = q.phitV;
           h.Ak = lin(Ma);
@@ -1064,37 +1103,28 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           h.C = GVb;
           run_gemm(ctx, h, "3e_ind_e_gv");
           timed(ctx, "3e_ind_energy", "e3i_kernel", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
-                [&] { e3i_energy(GWb, GVb, M, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
+                [&] { e3i_energy(GWb, GVb, Mr, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
         }
       }
       // ---- 3e indirect Fock (indirect_utils.cpp:138-345)
-      if (calc_fock) {
-        double* phitW = ctx->buf("phitW", size_t(M) * a * DW);
-        GemmDesc p;  // PhiT_W = S+ Phi_W[:, act]
-        p.M = M;
-        p.N = a * DW;
-        p.K = M;
-        p.A = P->Sp.p;
-        p.Am = lin(1);
-        p.Ak = lin(M);
-        p.B = q.phiW + size_t(M) * c;
-        p.Bk = lin(1);
-        p.Bn = two(uint32_t(a), M, int64_t(M) * o);
-        p.C = phitW;
-        p.Cm = lin(1);
-        p.Cn = lin(M);
-        run_gemm(ctx, p, "3e_abs_apply");
-        double* L = ctx->buf("blk2", MN * std::max(DW, DV));
-        const int64_t MNo = int64_t(MN) * o;
+      if (calc_fock && Mr > 0) {
+        double* phitW = ctx->buf("phitW", size_t(Mr) * a * DW);
+        timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DW, 16.0 * Mr * a * DW, [&] {
+          scale_rows(q.psiW + size_t(Mr) * c, Mr, a, DW, Mr, int64_t(Mr) * o, P->dinv.p, phitW, st);
+        });
+        const size_t MrN = size_t(Mr) * N;
+        double* L = ctx->buf("blk2", MrN * std::max(DW, DV));
+        const int64_t MNo = int64_t(MrN) * o;
         const size_t cap = ctx->block_cap;
         const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
         double* Kt = ctx->buf("blk1", size_t(MNo) * nj);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
         double* X2sT = ctx->buf("blk3", size_t(ao) * std::max(DW, DV));
-        // fock_k's R needs Q~[mu', i, alpha | j] = sum_B PhiT_V[mu', j, B] sV_B Phi_V[alpha, c+i, B]
-        // (indirect_utils.cpp:199-259), which is the first pass's K~[mu', (alpha, c+i) | j]: R and
-        // fock_k are formed from each K~ block of that pass before it is overwritten
+        // fock_k's R needs Q~[m, i, alpha | j] = sum_B PhiT_V[m, j, B] sV_B Phi_V[alpha, c+i, B]
+        // (indirect_utils.cpp:199-259), which is the first pass's K~[m, (alpha, c+i) | j]: R and
+        // fock_k are formed from each K~ block of that pass before it is overwritten.  The ABS
+        // index runs over the Mr rotated rows m throughout.
         double* R = ctx->buf("blk4", size_t(DW) * N * nj);
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
@@ -1107,52 +1137,52 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
           for (int j0 = 0; j0 < a; j0 += nj) {
             const int jn = std::min(nj, a - j0);
-            GemmDesc g;  // K~[mu', (beta, k) | j] = sum_B PhiT_Y[mu', j, B] sY_B Phi_Y[beta, k, B]
-            g.M = M;
+            GemmDesc g;  // K~[m, (beta, k) | j] = sum_B PhiT_Y[m, j, B] sY_B Phi_Y[beta, k, B]
+            g.M = Mr;
             g.N = N * o;
             g.K = DY;
             g.batch = jn;
-            g.A = phitY + size_t(M) * j0;
+            g.A = phitY + size_t(Mr) * j0;
             g.Am = lin(1);
-            g.Ak = lin(int64_t(M) * a);
-            g.Ab = M;
+            g.Ak = lin(int64_t(Mr) * a);
+            g.Ab = Mr;
             g.ks = Y->sign.p;
             g.Km = lin(1);
             g.B = phiY;
-            g.Bk = lin(int64_t(M) * o);
-            g.Bn = two(uint32_t(N), 1, M);
+            g.Bk = lin(int64_t(N) * o);
+            g.Bn = lin(1);
             g.C = Kt;
             g.Cm = lin(1);
-            g.Cn = lin(M);
+            g.Cn = lin(Mr);
             g.Cb = MNo;
             run_gemm(ctx, g, "3e_ind_fock_kt");
-            GemmDesc h;  // L[(mu', beta), A] += sum_{(k, j)} K~[(mu', beta), k | j] X2sT[A, j + a k]
-            h.M = int(MN);
+            GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
+            h.M = int(MrN);
             h.N = DX;
             h.K = o * jn;
             h.beta = j0 == 0 ? 0.0 : 1.0;
             h.A = Kt;
             h.Am = lin(1);
-            h.Ak = lin(int64_t(MN));
+            h.Ak = lin(int64_t(MrN));
             h.B = X2sT + int64_t(DX) * j0;
             h.Bk = two(uint32_t(o), int64_t(DX) * a, DX);
             h.Bn = lin(1);
             h.C = L;
             h.Cm = lin(1);
-            h.Cn = lin(int64_t(MN));
+            h.Cn = lin(int64_t(MrN));
             run_gemm(ctx, h, "3e_ind_fock_l");
             if (pass == 0) {
-              GemmDesc r;  // R[A, alpha | j] = sum_{(mu', i)} Phi_W[mu', c+i, A] K~[mu', (alpha, c+i) | j]
+              GemmDesc r;  // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j]
               r.M = DW;
               r.N = N;
-              r.K = M * a;
+              r.K = Mr * a;
               r.batch = jn;
-              r.A = q.phiW + size_t(M) * c;
-              r.Am = lin(int64_t(M) * o);
+              r.A = q.psiW + size_t(Mr) * c;
+              r.Am = lin(int64_t(Mr) * o);
               r.Ak = lin(1);
-              r.B = Kt + MN * size_t(c);
-              r.Bk = two(uint32_t(M), 1, int64_t(MN));
-              r.Bn = lin(M);
+              r.B = Kt + MrN * size_t(c);
+              r.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
+              r.Bn = lin(Mr);
               r.Bb = MNo;
               r.C = R;
               r.Cm = lin(1);
@@ -1165,9 +1195,9 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               f.K = DW * jn;
               f.alpha = k1;
               f.beta = 1.0;
-              f.A = q.phiW + size_t(M) * (c + j0);
+              f.A = q.phiW + size_t(N) * (c + j0);
               f.Am = lin(1);
-              f.Ak = two(uint32_t(DW), int64_t(M) * o, M);
+              f.Ak = two(uint32_t(DW), int64_t(N) * o, N);
               f.ks = W->sign.p;
               f.Km = two(uint32_t(DW), 1, 0);
               f.B = R;
@@ -1180,48 +1210,48 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             }
           }
           if (pass == 0 && mode != kAdjoint) {
-            // E3_ind = e k1 sum_{mu', i, A} Phi_W[mu', c+i, A] sum_beta L[mu', beta, A] Ca[beta, i]:
-            // L holds sum_{j,k,B} PhiT_V[mu', j, B] sV_B Phi_V[beta, k, B] W2s[(j, k), A], and Ca^T
+            // E3_ind = e k1 sum_{m, i, A} Psi_W[m, c+i, A] sum_beta L[m, beta, A] Ca[beta, i]:
+            // L holds sum_{j,k,B} PhiT_V[m, j, B] sV_B Phi_V[beta, k, B] W2s[(j, k), A], and Ca^T
             // over beta turns sV_B Phi_V[beta, k, B] into V2s[(i, k), B] (indirect_utils.cpp:19-133)
-            double* Lc = ctx->buf("blk5", size_t(M) * a * DW);
+            double* Lc = ctx->buf("blk5", size_t(Mr) * a * DW);
             GemmDesc g;
-            g.M = M;
+            g.M = Mr;
             g.N = a;
             g.K = N;
             g.batch = DW;
             g.A = L;
             g.Am = lin(1);
-            g.Ak = lin(M);
-            g.Ab = int64_t(MN);
+            g.Ak = lin(Mr);
+            g.Ab = int64_t(MrN);
             g.B = q.Co + size_t(N) * c;
             g.Bk = lin(1);
             g.Bn = lin(N);
             g.C = Lc;
             g.Cm = lin(1);
-            g.Cn = lin(M);
-            g.Cb = int64_t(M) * a;
+            g.Cn = lin(Mr);
+            g.Cb = int64_t(Mr) * a;
             run_gemm(ctx, g, "3e_ind_e_lc");
             DotDesc dd;
-            dd.x = q.phiW + size_t(M) * c;
+            dd.x = q.psiW + size_t(Mr) * c;
             dd.y = Lc;
-            dd.n = int64_t(M) * a * DW;
-            dd.ld = int64_t(M) * a;
-            dd.xs = int64_t(M) * o;
-            dd.ys = int64_t(M) * a;
+            dd.n = int64_t(Mr) * a * DW;
+            dd.ld = int64_t(Mr) * a;
+            dd.xs = int64_t(Mr) * o;
+            dd.ys = int64_t(Mr) * a;
             dot(dd, e * k1, Eacc + UW12G_E3_INDIRECT, 1, partial, st);
           }
-          GemmDesc f;  // F += k1 sum_{mu', A} Gamma_X[mu', alpha, A] L[mu', beta, A]
+          GemmDesc f;  // F += k1 sum_{m, A} Gt_X[m, alpha, A] L[m, beta, A]
           f.M = N;
           f.N = N;
-          f.K = M * DX;
+          f.K = Mr * DX;
           f.alpha = k1;
           f.beta = 1.0;
-          f.A = X->gamma.p;
-          f.Am = lin(M);
-          f.Ak = two(uint32_t(M), 1, int64_t(MN));
+          f.A = X->gt.p;
+          f.Am = lin(Mr);
+          f.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
           f.B = L;
-          f.Bk = two(uint32_t(M), 1, int64_t(MN));
-          f.Bn = lin(M);
+          f.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
+          f.Bn = lin(Mr);
           f.C = Fraw + NN * s;
           f.Cm = lin(1);
           f.Cn = lin(N);
@@ -1361,30 +1391,33 @@ int uw12g_setup_abs_projectors(uw12g_context* ctx, const double* S, size_t n_ao,
         inv.push_back(1.0 / w[i]);
       }
     const int K = int(keep.size());
+    static std::atomic<uint64_t> next_id{0};
+    P->id = ++next_id;
+    P->r = K;
     P->Sp.get(Mz * Mz);
+    P->U.get(Mz * std::max(K, 1));
+    P->dinv.get(std::max(K, 1));
     if (K == 0) {
       fill(P->Sp.p, int64_t(Mz * Mz), 0.0, s);
     } else {
       int* keep_d = nullptr;
       UW12G_HIP(hipMalloc(&keep_d, sizeof(int) * K));
-      DBuf on, iv, Vk, Vs;
+      DBuf on, Vs;
       on.get(K);
-      iv.get(K);
-      Vk.get(Mz * K);
       Vs.get(Mz * K);
       UW12G_HIP(hipMemcpyAsync(keep_d, keep.data(), sizeof(int) * K, hipMemcpyHostToDevice, s));
       UW12G_HIP(hipMemcpyAsync(on.p, ones.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
-      UW12G_HIP(hipMemcpyAsync(iv.p, inv.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
-      gather_scale_cols(Vd.p, M, keep_d, on.p, Vk.p, K, s);
-      gather_scale_cols(Vd.p, M, keep_d, iv.p, Vs.p, K, s);
-      GemmDesc g;  // S+ = V_k diag(1/s) V_k^T
+      UW12G_HIP(hipMemcpyAsync(P->dinv.p, inv.data(), sizeof(double) * K, hipMemcpyHostToDevice, s));
+      gather_scale_cols(Vd.p, M, keep_d, on.p, P->U.p, K, s);
+      gather_scale_cols(Vd.p, M, keep_d, P->dinv.p, Vs.p, K, s);
+      GemmDesc g;  // S+ = U diag(1/s) U^T
       g.M = M;
       g.N = M;
       g.K = K;
       g.A = Vs.p;
       g.Am = lin(1);
       g.Ak = lin(M);
-      g.B = Vk.p;
+      g.B = P->U.p;
       g.Bk = lin(M);
       g.Bn = lin(1);
       g.C = P->Sp.p;

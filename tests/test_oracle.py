@@ -10,7 +10,10 @@
   alpha/beta orbitals equals closed shell, no active orbitals gives zero,
   energy-only equals the Fock path energy, indirect flag irrelevant at k1 = 0,
   one-electron self-interaction cancellation;
-* the fallback BLAS/eigensolver path equals the MKL path.
+* the fallback BLAS/eigensolver path equals the MKL path;
+* the extended-precision evaluator (oracle/extended.py) equals the brute force
+  on well-conditioned inputs, and on the s-Gaussian recipe measures the FP64
+  rounding floor of the reference-order evaluation.
 """
 import os
 
@@ -19,7 +22,7 @@ import pytest
 
 from helpers import TERMS
 from libuw12_amd import synth
-from oracle import brute, cpu
+from oracle import brute, cpu, extended
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -187,6 +190,34 @@ def test_recipe_conditioning_floor():
     floor = max(abs(e1[k] - e2[k]) for k in TERMS)
     assert floor < 2e-9
     assert np.abs(F1[0] - F2[0]).max() < 2e-8
+
+
+@pytest.mark.parametrize('n_occ,n_active', CASES)
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (0.3, 0.0)])
+def test_extended_matches_brute_force(n_occ, n_active, kappa):
+    """Well-conditioned inputs: FP64 brute force is accurate, so the long-double
+    evaluation must reproduce it to FP64 rounding."""
+    sys_ = synth.random_system(7, 10, 12, n_occ, n_active, seed=5 + sum(n_occ))
+    args = (sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], n_active) + kappa
+    eb, ex = brute.energy_terms(*args), extended.energy_terms(*args)
+    scale = max(abs(eb[k]) for k in TERMS)
+    for k in TERMS:
+        assert abs(ex[k] - eb[k]) <= 1e-13 * scale, (k, ex[k], eb[k])
+
+
+@pytest.mark.parametrize('name', ['c1', 'm6u'])
+def test_oracle_recipe_rounding_floor(name):
+    """s-Gaussian recipe: the reference-order FP64 evaluation stays within its
+    rounding floor of the long-double value.  The 2e and 4e terms carry no ABS
+    projector and agree to ~1e-12; the 3e terms multiply through S+ formed
+    explicitly (|S+| ~ 1/1.1e-6) and sit ~1e-9 off (2.7e-9 Eh at c1)."""
+    sys_ = synth.make_system(name)
+    ex = extended.energy_terms(sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], sys_['n_active'], 1.0, 0.5)
+    eo, _ = oracle(sys_, 1.0, 0.5, fock=False)
+    for k in ('e2_direct', 'e2_indirect', 'e4_direct', 'e4_indirect'):
+        assert abs(eo[k] - ex[k]) <= 1e-11, (k, eo[k], ex[k])
+    for k in ('e3_direct', 'e3_indirect'):
+        assert abs(eo[k] - ex[k]) <= 1e-8, (k, eo[k], ex[k])
 
 
 @pytest.fixture(scope='module')
