@@ -1,9 +1,16 @@
 #!/usr/bin/env python
-"""UW12 energy + Fock evaluations/s on one MI355X (plain FP64 form).
+"""UW12 energy + Fock evaluations/s on one MI355X.
 
 One step = one full uw12_fock evaluation (2-, 3- and 4-electron energy terms
 and the Fock matrix) of a synthetic s-Gaussian cluster whose density-fitted
 integrals are already resident in HBM (set up once, as an SCF caller would).
+
+The timed form is the mixed one by default (FP32 matrix cores for the large
+intermediate tensors of the Fock terms, FP64 for the orbital transforms, the
+energies and every accumulation into the Fock matrix).  The plain FP64 path is
+timed beside it on the same inputs, and the mixed result is gated against it:
+`accuracy_gate` reports the largest energy-term and Fock deviations and
+whether both are within 1e-6.  `--form fp64` times the plain path instead.
 
 Prints one JSON line (see the keys below).  --gpus > 1 is rejected: this
 build measures one GPU.
@@ -21,7 +28,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = 'UW12 E+Fock evals/s, mixed vs FP64 on 1 MI355X at ≤1e-6 Eh; MFMA % peak'
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix peak (AMD published)
+PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak (AMD published)
 PEAK_HBM_GBS = 8000.0
+GATE = 1e-6  # energy (Eh, per term and total) and Fock max-abs bar of the reduced-precision forms
+TERM_KEYS = ('e2_direct', 'e2_indirect', 'e3_direct', 'e3_indirect', 'e4_direct', 'e4_indirect', 'total')
 
 
 def flop_model(N, Nd, Nr, occ, ss):
@@ -79,8 +89,11 @@ def roofline_from_profile(records):
                           for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])}
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
-        roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=PEAK_FP64_TFLOPS, unit='TFLOP/s',
-                    frac=round(achieved / PEAK_FP64_TFLOPS, 4), traffic=None,
+        # gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>: PREC 1 runs on the FP32 matrix cores
+        peak = PEAK_FP32_TFLOPS if dom.startswith('gemm_f64_kernel<') and dom.rstrip('>').endswith(',1') \
+            else PEAK_FP64_TFLOPS
+        roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=peak, unit='TFLOP/s',
+                    frac=round(achieved / peak, 4), traffic=None,
                     launches=n, avg_launch_us=round(1e3 * ms / n, 2),
                     algorithmic_flops_per_launch=fl / n, share_of_gpu_time=round(ms / total_ms, 3))
     else:
@@ -91,10 +104,14 @@ def roofline_from_profile(records):
     return roof, table
 
 
-def measured_traffic(kernel, config):
+def traffic_profile(config, form):
+    return f'profiles/gemm_f64_{config}_{form}_traffic.md'
+
+
+def measured_traffic(kernel, config, form):
     """HBM-side bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    this workload (profiles/gemm_f64_<config>_fp64_traffic.md), or None when there is none."""
-    path = os.path.join(ROOT, 'profiles', f'gemm_f64_{config}_fp64_traffic.md')
+    this workload (profiles/gemm_f64_<config>_<form>_traffic.md), or None when there is none."""
+    path = os.path.join(ROOT, traffic_profile(config, form))
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -109,7 +126,7 @@ def cpu_baseline(system, k0, k1, pair_samples=(1, 4), full=False, gpu_result=Non
     same inputs, with the same-spin pair loops on two uniform samples of the outer
     pair index (pair_samples indices each), extrapolated linearly.  With
     full=True the whole evaluation runs once (no sampling) and its energy pieces
-    and Fock are compared with gpu_result = (terms, Fock list)."""
+    and Fock are compared with each GPU form in gpu_result = {form: (terms, Fock list)}."""
     from oracle import cpu
     threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
     t = time.perf_counter()
@@ -128,14 +145,11 @@ def cpu_baseline(system, k0, k1, pair_samples=(1, 4), full=False, gpu_result=Non
         out = dict(value=round(1.0 / t_full, 6), unit='evals/s', cores=threads, kind='port',
                    sample='one full oracle evaluate on the same inputs (no sampling); setup (metric eig, J3) excluded',
                    seconds_per_eval=round(t_full, 2), blas=cpu.blas_kind())
-        if gpu_result is not None:
-            eg, Fg = gpu_result
-            out['parity'] = {'term_diff_eh': {k: eg[k] - eo[k] for k in ('e2_direct', 'e2_indirect', 'e3_direct',
-                                                                        'e3_indirect', 'e4_direct', 'e4_indirect',
-                                                                        'total')},
-                             'oracle_energy': eo['total'],
-                             'fock_max_abs_diff': max(float(np.abs(a - b).max()) for a, b in zip(Fg, Fo)),
-                             'fock_max_abs': max(float(np.abs(b).max()) for b in Fo)}
+        if gpu_result:
+            out['parity'] = {'oracle_energy': eo['total'], 'fock_max_abs': max(float(np.abs(b).max()) for b in Fo)}
+            for form, (eg, Fg) in gpu_result.items():
+                out['parity'][form] = {'term_diff_eh': {k: eg[k] - eo[k] for k in TERM_KEYS},
+                                       'fock_max_abs_diff': max(float(np.abs(a - b).max()) for a, b in zip(Fg, Fo))}
         return out
     # The same-spin pair loops run on every s-th outer pair index; timing the whole
     # evaluation at two strides s1 > s2 separates the fixed part T0 from the cost p
@@ -183,6 +197,8 @@ def main():
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--config', default='c2', choices=['c1', 'c2', 'c3', 'c4'])
     ap.add_argument('--kappa', default='1.0,0.5')
+    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32'],
+                    help='precision form of the timed steps (the plain FP64 path is always run beside it)')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
                     help='auto: sampled oracle timing; full: one complete oracle evaluation, with parity report')
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
@@ -236,13 +252,15 @@ def main():
     print(f'[bench] device setup {t_setup:.1f}s (kept DF directions W={W.n_kept} V={V.n_kept} WV={WV.n_kept})',
           file=sys.stderr, flush=True)
     indirect = abs(k1) > 4 * np.finfo(float).eps
+    prec = {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED}[args.form]
 
-    def step():
+    def evaluation(precision):
         return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
-                           indirect_term=indirect, calculate_fock=True, scale_opp_spin=k0, scale_same_spin=k1)
+                           indirect_term=indirect, calculate_fock=True, scale_opp_spin=k0, scale_same_spin=k1,
+                           precision=precision)
 
     for i in range(args.warmup):
-        step()
+        evaluation(prec)
         print(f'[bench] warmup {i + 1}/{args.warmup} done', file=sys.stderr, flush=True)
     try:
         import torch
@@ -254,7 +272,7 @@ def main():
     sync()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        terms, F = step()
+        terms, F = evaluation(prec)
         records.extend(ctx.profile())
         print(f'[bench] step {i + 1}/{args.steps} done', file=sys.stderr, flush=True)
     sync()
@@ -262,33 +280,62 @@ def main():
     ctx.set_profiling(False)
     value = args.steps / elapsed
     roof, table = roofline_from_profile(records)
-    roof['traffic'] = measured_traffic(roof['kernel'], args.config)
+    roof['traffic'] = measured_traffic(roof['kernel'], args.config, args.form)
     if roof['traffic'] is not None:
-        roof['traffic_source'] = f'profiles/gemm_f64_{args.config}_fp64_traffic.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)'
+        roof['traffic_source'] = f'{traffic_profile(args.config, args.form)} (rocprofv3 FETCH_SIZE + WRITE_SIZE)'
     gpu_flops = sum(r[2] for r in records) / args.steps
     model = flop_model(N, Nd, Nr, occ_counts, indirect)
     if args.profile_table:
         print(json.dumps(table, indent=1), file=sys.stderr)
-    # precision form on the same inputs (one untimed warm-up, one timed evaluation)
-    step32 = lambda: u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],  # noqa: E731
-                                 indirect_term=indirect, calculate_fock=True, scale_opp_spin=k0,
-                                 scale_same_spin=k1, precision=1)
-    step32()
-    sync()
-    t = time.perf_counter()
-    terms32, F32 = step32()
-    sync()
-    t32 = time.perf_counter() - t
-    fp32 = {'evals_per_s': round(1.0 / t32, 6), 'speedup_vs_fp64': round(elapsed / args.steps / t32, 3),
-            'term_error_eh': {k: terms32[k] - terms[k] for k in ('e2_direct', 'e2_indirect', 'e3_direct',
-                                                                 'e3_indirect', 'e4_direct', 'e4_indirect', 'total')},
-            'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(F32, F))}
-    print(f'[bench] FP32 form: {1e3 * t32:.1f} ms, total energy error {fp32["term_error_eh"]["total"]:.3e} Eh',
-          file=sys.stderr, flush=True)
+
+    def deviation(tr, Fr, t_ref, F_ref):
+        return {'term_error_eh': {k: tr[k] - t_ref[k] for k in TERM_KEYS},
+                'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(Fr, F_ref))}
+
+    def timed_form(precision, name, reps=3):
+        evaluation(precision)
+        sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            tr, Fr = evaluation(precision)
+        sync()
+        tp = (time.perf_counter() - t) / reps
+        print(f'[bench] {name} form: {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
+        return tr, Fr, tp
+
+    # the plain FP64 path on the same inputs: reference for the reduced-precision forms
+    if args.form == 'fp64':
+        terms64, F64, t64 = terms, F, elapsed / args.steps
+    else:
+        terms64, F64, t64 = timed_form(u.FP64, 'FP64')
+    fp64 = {'evals_per_s': round(1.0 / t64, 6), 'ms_per_eval': round(1e3 * t64, 3), 'energy': terms64['total']}
+    gate = None
+    if args.form != 'fp64':
+        dev = deviation(terms, F, terms64, F64)
+        worst_e = max(abs(v) for v in dev['term_error_eh'].values())
+        gate = dict(dev, bar=GATE, max_energy_error_eh=worst_e,
+                    met=bool(worst_e <= GATE and dev['fock_max_abs_error'] <= GATE),
+                    speedup_vs_fp64=round(t64 * value, 3))
+        print(f'[bench] {args.form} vs FP64: max energy error {worst_e:.3e} Eh, Fock max-abs error '
+              f'{dev["fock_max_abs_error"]:.3e}, gate {"met" if gate["met"] else "NOT met"}',
+              file=sys.stderr, flush=True)
+    others = {}
+    for name, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
+        if name == args.form:
+            continue
+        tr, Fr, tp = timed_form(precision, name)
+        others[f'{name}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
+                                      speedup_vs_fp64=round(t64 / tp, 3))
     cpu = None
     if args.cpu_baseline in ('auto', 'full') and not stream:
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
-        cpu = cpu_baseline(system, k0, k1, full=args.cpu_baseline == 'full', gpu_result=(terms, F))
+        cpu = cpu_baseline(system, k0, k1, full=args.cpu_baseline == 'full',
+                           gpu_result={'fp64': (terms64, F64), args.form: (terms, F)})
+    form_text = {'fp64': 'plain FP64',
+                 'fp32': 'FP32 form (every contraction on FP32 matrix cores, FP64 storage and reductions)',
+                 'mixed': 'mixed form (FP32 matrix cores for the Fock intermediates K~, L, R, Xi, G, X_i/Y_i; FP64 '
+                          'for the orbital transforms, energies and Fock accumulation), gated at <=1e-6 Eh per '
+                          'term and <=1e-6 Fock max-abs against the plain FP64 path of the same run'}[args.form]
     out = {
         'metric': METRIC,
         'value': round(value, 6),
@@ -300,20 +347,22 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'f64',
+        'dtype': {'fp64': 'f64', 'fp32': 'f32', 'mixed': 'f32+f64'}[args.form],
         'data': 'synthetic (seeded s-Gaussian cluster, analytic Boys-F0 integrals)',
         'config': {'workload': f'{args.config}: UW12 energy+Fock, n_ao={N} n_df={Nd} n_ri={Nr} n_occ={occ_counts}, '
-                               f'kappa=({k0},{k1}), plain FP64',
-                   'form': 'plain-fp64', 'kappa': [k0, k1], 'parallelism': 'single-gpu'},
+                               f'kappa=({k0},{k1}), {form_text}',
+                   'form': args.form, 'kappa': [k0, k1], 'parallelism': 'single-gpu'},
         'roofline': roof,
         'cpu_baseline': cpu,
+        'accuracy_gate': gate,
+        'fp64_form': fp64,
         'energy': terms['total'],
         'gpu_tflops_executed': round(gpu_flops * value / 1e12, 3),
         'model_flops_per_eval': model,
         'executed_flops_per_eval': gpu_flops,
         'setup_s': round(t_setup, 2),
-        'fp32_form': fp32,
     }
+    out.update(others)
     print(json.dumps(out), flush=True)
 
 

@@ -35,8 +35,12 @@ typedef struct uw12g_abs_projectors uw12g_abs_projectors;
 /* Arithmetic of the bulk contractions.  FP64 is the plain form.  UW12G_FP32 runs
  * every contraction of the evaluation on FP32 matrix cores (operands rounded to
  * FP32, FP32 sums over k ranges of at most 4096, ranges combined in FP64);
- * storage, the metric eigensolves and the energy reductions stay FP64. */
-typedef enum { UW12G_FP64 = 0, UW12G_FP32 = 1 } uw12g_precision;
+ * storage, the metric eigensolves and the energy reductions stay FP64.
+ * UW12G_MIXED runs the large intermediate tensors of the Fock terms that way
+ * and keeps in FP64 the contractions whose results enter the energies directly
+ * (orbital transforms, t, X, the energy-only products) and those that
+ * accumulate into the Fock matrix. */
+typedef enum { UW12G_FP64 = 0, UW12G_FP32 = 1, UW12G_MIXED = 2 } uw12g_precision;
 
 /* Term order of terms_out[] */
 enum {

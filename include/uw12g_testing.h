@@ -41,6 +41,10 @@ int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, do
  * this context; small values force many shell groups (tests). */
 int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles);
 
+/* Comma-separated stage names whose contractions the mixed form also runs in
+ * FP64 on this context (error attribution); NULL or "" clears. */
+int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);
+
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);
 

@@ -26,6 +26,7 @@ EXPORTS = (
     'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
     'uw12g_context_profile',
     'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm', 'uw12g_test_set_stream_cap',
+    'uw12g_test_set_exact_stages',
 )
 
 
@@ -86,6 +87,7 @@ def load():
                                   vp, Map2, Map2, ctypes.c_longlong,
                                   vp, Map2, ctypes.c_int, dp]
     L.uw12g_test_set_stream_cap.argtypes = [vp, ctypes.c_size_t]
+    L.uw12g_test_set_exact_stages.argtypes = [vp, ctypes.c_char_p]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]
     L.uw12g_test_map2.restype = ctypes.c_longlong
     _lib = L

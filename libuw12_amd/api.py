@@ -13,6 +13,8 @@ from . import _lib
 from ._lib import UW12Error, check, dp, szp, vp
 
 TERMS = ('e2_direct', 'e2_indirect', 'e3_direct', 'e3_indirect', 'e4_direct', 'e4_indirect')
+# uw12g_precision values (include/uw12g.h)
+FP64, FP32, MIXED = 0, 1, 2
 
 
 def _f64(a):

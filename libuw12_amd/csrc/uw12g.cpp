@@ -164,6 +164,7 @@ struct uw12g_context {
   // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
   size_t block_cap = size_t(1) << 28;
   int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
+  std::vector<std::string> exact_extra;  // stage names also kept FP64 in the mixed form (tests)
   size_t stream_cap = size_t(1) << 25;  // doubles per host staging buffer of the per-shell setup
   // Differentiation form: arguments of the last forward call.  Its
   // intermediates live in the named buffers until the next fused or forward
@@ -262,11 +263,19 @@ void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)); }
 
 // GEMM launch (timed per launch when profiling is on) in the context's
 // current contraction precision.
-void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage) {
+// `exact` contractions stay FP64 in the mixed form: those whose results feed
+// the energies directly (orbital transforms, t, X, the energy-only products)
+// and those that accumulate into the Fock matrix (long sums of terms that
+// cancel).  The mixed form runs the rest -- the large intermediate tensors of
+// the Fock terms (K~, L, R, Xi, G, X_i / Y_i) -- on FP32 matrix cores.
+void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact = false) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   ctx->prof.stage = stage;
   GemmDesc e = d;
-  e.prec = ctx->prec;
+  if (ctx->prec == UW12G_MIXED && !exact)
+    for (const std::string& x : ctx->exact_extra)
+      if (x == stage) exact = true;
+  e.prec = ctx->prec == UW12G_MIXED ? (exact ? 0 : 1) : ctx->prec;
   gemm_f64(e, ctx->stream, &ctx->gemm_ws, &ctx->prof);
 }
 
@@ -642,7 +651,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       g.C = DCa + NN * s;
       g.Cm = lin(1);
       g.Cn = lin(N);
-      run_gemm(ctx, g, "2e_density");
+      run_gemm(ctx, g, "2e_density", true);
     }
     // X_D[s]_A = sum_j (jj|A~)  (integrals.cpp:347-378); HBM-bound pass over Gamma_WV
     double* XD = ctx->buf("XD", size_t(std::max(DWV, 1)) * ns);
@@ -695,7 +704,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         g.Cm = lin(1);
         g.Cn = lin(N);
         g.Cb = int64_t(N) * a;
-        run_gemm(ctx, g, "2e_phi");
+        run_gemm(ctx, g, "2e_phi", true);
         GemmDesc h;  // O2[i, (j,A)] = sum_mu Ca[mu,i] P1[mu,(j,A)]
         h.M = a;
         h.N = a * DWV;
@@ -709,7 +718,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         h.C = O2;
         h.Cm = lin(1);
         h.Cn = lin(a);
-        run_gemm(ctx, h, "2e_o2");
+        run_gemm(ctx, h, "2e_o2", true);
         DotDesc dd;  // E -= 1/2 e k1 sum_A s_A |T_A|^2  (two_electron.cpp:112-114)
         dd.x = O2;
         dd.n = int64_t(a) * a * DWV;
@@ -735,7 +744,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           f.C = Fraw + NN * s;
           f.Cm = lin(1);
           f.Cn = lin(N);
-          run_gemm(ctx, f, "2e_ind_fock");
+          run_gemm(ctx, f, "2e_ind_fock", true);
         }
       }
     }
@@ -778,7 +787,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       g.Cm = lin(1);
       g.Cn = lin(N);
       g.Cb = int64_t(N) * o;
-      run_gemm(ctx, g, "phi_transform");
+      run_gemm(ctx, g, "phi_transform", true);
       GemmDesc gp = g;  // Psi[m, k, A] = sum_alpha (U^T Gamma)[m, alpha, A] Co[alpha, k]
       gp.M = Mr;
       gp.A = X->gt.p;
@@ -787,7 +796,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       gp.C = psi;
       gp.Cn = lin(Mr);
       gp.Cb = int64_t(Mr) * o;
-      run_gemm(ctx, gp, "psi_transform");
+      run_gemm(ctx, gp, "psi_transform", true);
       GemmDesc h;  // O2[i, (k, A)] = sum_mu Ca[mu, i] Phi[mu, (k, A)]
       h.M = a;
       h.N = o * X->D;
@@ -801,7 +810,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       h.C = o2;
       h.Cm = lin(1);
       h.Cn = lin(a);
-      run_gemm(ctx, h, "o2_transform");
+      run_gemm(ctx, h, "o2_transform", true);
     }
     GemmDesc t;  // t = O2_W^T O2_V  (four_electron_utils.cpp:33-49)
     t.M = DW;
@@ -816,7 +825,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     t.C = q.t;
     t.Cm = lin(1);
     t.Cn = lin(DW);
-    run_gemm(ctx, t, "4e_tab");
+    run_gemm(ctx, t, "4e_tab", true);
     // S+ Phi_V[:, c+i, B] = U PhiT_V[:, i, B] with PhiT_V[m, i, B] = (1/s_m) Psi_V[m, c+i, B]
     timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DV, 16.0 * Mr * a * DV, [&] {
       scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, q.phitV, st);
@@ -834,7 +843,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     x.C = q.x;
     x.Cm = lin(1);
     x.Cn = lin(DW);
-    run_gemm(ctx, x, "3e_xab");
+    run_gemm(ctx, x, "3e_xab", true);
   }
   // direct energies (four_electron.cpp:118-119, direct_utils.cpp:136-157)
   for (int s = 0; s < ns; ++s)
@@ -898,7 +907,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       f.C = Fraw + NN * s;
       f.Cm = lin(1);
       f.Cn = lin(N);
-      run_gemm(ctx, f, "direct_fock_F");
+      run_gemm(ctx, f, "direct_fock_F", true);
       // t~ dX/dD (direct_utils.cpp:242-347): F -= sum_AB Tb_AB Gamma_W[:, a, A]^T S+ Gamma_V[:, b, B]
       // = sum_{m, A} Gt_W[m, a, A] (1/s_m) Xi[m, b, A], Xi = Gt_V Tb^T over B
       const size_t MrN = size_t(Mr) * N;
@@ -937,7 +946,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         a3.C = Fraw + NN * s;
         a3.Cm = lin(1);
         a3.Cn = lin(N);
-        run_gemm(ctx, a3, "3e_ttdx_fock");
+        run_gemm(ctx, a3, "3e_ttdx_fock", true);
       }
     }
   }
@@ -972,7 +981,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           g.C = op == 0 ? W4 : V4;
           g.Cm = lin(1);
           g.Cn = lin(ao);
-          run_gemm(ctx, g, "4e_ind_x4");
+          run_gemm(ctx, g, "4e_ind_x4", true);
         }
         timed(ctx, "4e_ind_energy", "e4i_kernel", 2.0 * ao * ao, 16.0 * ao * ao,
               [&] { e4i_energy(W4, V4, a, o, -0.5 * e * k1, Eacc + UW12G_E4_INDIRECT, partial, st); });
@@ -1037,7 +1046,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         gf.C = Fraw + NN * s;
         gf.Cm = lin(1);
         gf.Cn = lin(N);
-        run_gemm(ctx, gf, "4e_ind_fock_f");
+        run_gemm(ctx, gf, "4e_ind_fock_f", true);
         if (c > 0) {
           const size_t nXY2 = size_t(N) * ao * o;
           double* X2b = ctx->buf("blk3", nXY2);
@@ -1068,7 +1077,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           hf.K = int(ao * o);
           hf.A = X2b;
           hf.B = Y2b;
-          run_gemm(ctx, hf, "4e_ind_fock_f");
+          run_gemm(ctx, hf, "4e_ind_fock_f", true);
         }
       }
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k.  With the
@@ -1094,14 +1103,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           g.C = GWb;
           g.Cm = lin(1);
           g.Cn = lin(Ma);
-          run_gemm(ctx, g, "3e_ind_e_gw");
+          run_gemm(ctx, g, "3e_ind_e_gw", true);
           GemmDesc h = g;  // GV[(m, j), (i, k)] = sum_B PhiT_V[m, j, B] V2s[(i, k), B]
           h.K = DV;
           h.A = q.phitV;
           h.Ak = lin(Ma);
           h.B = V2s + int64_t(a) * k0;
           h.C = GVb;
-          run_gemm(ctx, h, "3e_ind_e_gv");
+          run_gemm(ctx, h, "3e_ind_e_gv", true);
           timed(ctx, "3e_ind_energy", "e3i_kernel", 2.0 * Ma * a * kn, 16.0 * Ma * a * kn,
                 [&] { e3i_energy(GWb, GVb, Mr, a, kn, k1 * e, Eacc + UW12G_E3_INDIRECT, partial, st); });
         }
@@ -1206,7 +1215,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               f.C = Fraw + NN * s;
               f.Cm = lin(1);
               f.Cn = lin(N);
-              run_gemm(ctx, f, "3e_ind_fock_k");
+              run_gemm(ctx, f, "3e_ind_fock_k", true);
             }
           }
           if (pass == 0 && mode != kAdjoint) {
@@ -1230,7 +1239,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             g.Cm = lin(1);
             g.Cn = lin(Mr);
             g.Cb = int64_t(Mr) * a;
-            run_gemm(ctx, g, "3e_ind_e_lc");
+            run_gemm(ctx, g, "3e_ind_e_lc", true);
             DotDesc dd;
             dd.x = q.psiW + size_t(Mr) * c;
             dd.y = Lc;
@@ -1255,7 +1264,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           f.C = Fraw + NN * s;
           f.Cm = lin(1);
           f.Cn = lin(N);
-          run_gemm(ctx, f, "3e_ind_fock_f");
+          run_gemm(ctx, f, "3e_ind_fock_f", true);
         }
       }
     }
@@ -1464,11 +1473,12 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision == UW12G_FP64 || precision == UW12G_FP32, "unsupported precision", -2);
+    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
+            -2);
     require(ctx != nullptr, "null context");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
-    PrecScope scope(ctx, precision == UW12G_FP32 ? 1 : 0);
+    PrecScope scope(ctx, int(precision));
     ++ctx->n_eval;
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
              scale_opp_spin, scale_same_spin, fock, tp, energy);
@@ -1481,13 +1491,14 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
                   int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                   double* terms_out, double* energy, unsigned long long* tape) {
   return guard([&] {
-    require(precision == UW12G_FP64 || precision == UW12G_FP32, "unsupported precision", -2);
+    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
+            -2);
     require(ctx != nullptr && tape != nullptr, "null argument");
     require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
     require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
-    PrecScope scope(ctx, precision == UW12G_FP32 ? 1 : 0);
+    PrecScope scope(ctx, int(precision));
     ++ctx->n_eval;
     ctx->tape.id = 0;  // no valid tape while the forward runs
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
@@ -1557,6 +1568,23 @@ int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles) {
   return guard([&] {
     require(ctx != nullptr && doubles > 0, "invalid argument");
     ctx->stream_cap = doubles;
+  });
+}
+
+int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes) {
+  return guard([&] {
+    require(ctx != nullptr, "null context");
+    ctx->exact_extra.clear();
+    std::string cur;
+    for (const char* c = prefixes ? prefixes : ""; ; ++c) {
+      if (*c == ',' || *c == '\0') {
+        if (!cur.empty()) ctx->exact_extra.push_back(cur);
+        cur.clear();
+        if (*c == '\0') break;
+      } else {
+        cur += *c;
+      }
+    }
   });
 }
 
