@@ -33,7 +33,8 @@ struct Params {
   GemmDesc d;
   int tiles_m, tiles_n, splits, kchunk;
   double* out;  // split-K partials (dense, M x N per split and batch) or nullptr
-  int remap;    // 1: spread consecutive tiles over XCDs in contiguous chunks
+  int64_t work;  // output tiles x k chunks x batches: one workgroup each
+  int64_t per;   // > 0: work items per XCD (contiguous ranges; the grid is 8 per)
   int fast;     // 1: buffer-load main path (fast_ok); every slab of every chunk is full
   int nfast;    // 1: tiles ordered with the column index fastest
   int kbase;    // first k of chunk 0
@@ -107,18 +108,26 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  int tile = blockIdx.x;
-  if (p.remap) {  // blockIdx -> XCD is round-robin over 8 XCDs; give each XCD a contiguous range
-    const int per = (p.tiles_m * p.tiles_n) >> 3;
-    tile = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  // Work item w = tile + tiles (split + splits b).  Workgroups go to the 8 XCDs
+  // round-robin by blockIdx; giving each XCD a contiguous range of w puts the
+  // tiles of one (k chunk, batch) -- which share its A and B panels -- on one
+  // XCD, so each panel is read from HBM into that XCD's L2 once.
+  int64_t w = blockIdx.x;
+  if (p.per > 0) {
+    w = int64_t(blockIdx.x & 7) * p.per + (blockIdx.x >> 3);
+    if (w >= p.work) return;  // tail of the last XCD's range (whole workgroup, before any barrier)
   }
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int tile = int(w % tiles);
+  const int64_t q = w / tiles;
+  const int split = int(q % p.splits);
+  const int b = int(q / p.splits);
   // column-fastest order for tall outputs: the tiles that share an A panel run
-  // together (on one XCD when remapped), so the panel is read from HBM once
+  // together, so the panel is read from HBM once
   const int tm = p.nfast ? tile / p.tiles_n : tile % p.tiles_m;
   const int tn = p.nfast ? tile % p.tiles_n : tile / p.tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
-  const int b = blockIdx.z;
-  const int k_begin = p.kbase + blockIdx.y * p.kchunk;
+  const int k_begin = p.kbase + split * p.kchunk;
   const int k_end = min(d.K, k_begin + p.kchunk);
 
   const double* __restrict__ Ap = d.A + d.Ab * b;
@@ -339,7 +348,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   // j + 16 (i / 4), i.e. C[m = .. + lr][n = .. + 4 lk + r].
   auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + (PREC == 0 ? lk + 4 * r : 4 * lk + r); };
   if (p.out) {
-    double* W = p.out + (int64_t(blockIdx.y) * d.batch + b) * int64_t(d.M) * d.N;
+    double* W = p.out + (int64_t(split) * d.batch + b) * int64_t(d.M) * d.N;
 #pragma unroll
     for (int i = 0; i < MB; ++i) {
       const int m = m0 + wm0 + 16 * i + lr;
@@ -579,10 +588,13 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   p.kbase = kbase;
   p.out = nullptr;
   if (splits > 1) p.out = ws->get(size_t(splits) * d.batch * size_t(d.M) * d.N, stream);
-  p.remap = (tiles % 8 == 0 && tiles >= 64) ? 1 : 0;
+  p.work = tiles * splits * int64_t(d.batch);
+  p.per = p.work >= 64 ? (p.work + 7) / 8 : 0;
+  const int64_t nblocks = p.per > 0 ? 8 * p.per : p.work;
+  if (nblocks > 0x7fffffff) throw Error(-2, "gemm grid too large");
   p.fast = fast ? 1 : 0;
   p.nfast = (p.tiles_n <= 16 && p.tiles_m >= 4 * p.tiles_n) ? 1 : 0;
-  dim3 grid(unsigned(tiles), unsigned(splits), unsigned(d.batch));
+  const dim3 grid{static_cast<unsigned>(nblocks)};
   const bool timing = prof && prof->on;
   ProfileRec rec;
   if (timing) prof->begin(rec, stream);
