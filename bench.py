@@ -198,7 +198,12 @@ def main():
     ap.add_argument('--config', default='c2', choices=['c1', 'c2', 'c3', 'c4'])
     ap.add_argument('--kappa', default='1.0,0.5')
     ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32'],
-                    help='precision form of the timed steps (the plain FP64 path is always run beside it)')
+                    help='precision form of the timed steps (the plain FP64 path is run beside it)')
+    ap.add_argument('--side-reps', type=int, default=3,
+                    help='timed evaluations of each side form (after one untimed one when more than 1)')
+    ap.add_argument('--side-forms', default='auto', choices=['auto', 'skip'],
+                    help='skip: no FP64 / FP32 / mixed evaluations beside the timed ones (no accuracy gate); '
+                         'for profiler runs that should see only the timed form')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
                     help='auto: sampled oracle timing; full: one complete oracle evaluation, with parity report')
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
@@ -292,8 +297,9 @@ def main():
         return {'term_error_eh': {k: tr[k] - t_ref[k] for k in TERM_KEYS},
                 'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(Fr, F_ref))}
 
-    def timed_form(precision, name, reps=3):
-        evaluation(precision)
+    def timed_form(precision, name, reps=max(1, args.side_reps)):
+        if reps > 1:
+            evaluation(precision)
         sync()
         t = time.perf_counter()
         for _ in range(reps):
@@ -304,13 +310,15 @@ def main():
         return tr, Fr, tp
 
     # the plain FP64 path on the same inputs: reference for the reduced-precision forms
-    if args.form == 'fp64':
+    side = args.side_forms == 'auto'
+    if args.form == 'fp64' or not side:
         terms64, F64, t64 = terms, F, elapsed / args.steps
     else:
         terms64, F64, t64 = timed_form(u.FP64, 'FP64')
-    fp64 = {'evals_per_s': round(1.0 / t64, 6), 'ms_per_eval': round(1e3 * t64, 3), 'energy': terms64['total']}
+    fp64 = {'evals_per_s': round(1.0 / t64, 6), 'ms_per_eval': round(1e3 * t64, 3),
+            'energy': terms64['total']} if side or args.form == 'fp64' else None
     gate = None
-    if args.form != 'fp64':
+    if args.form != 'fp64' and side:
         dev = deviation(terms, F, terms64, F64)
         worst_e = max(abs(v) for v in dev['term_error_eh'].values())
         gate = dict(dev, bar=GATE, max_energy_error_eh=worst_e,
@@ -321,7 +329,7 @@ def main():
               file=sys.stderr, flush=True)
     others = {}
     for name, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
-        if name == args.form:
+        if name == args.form or not side:
             continue
         tr, Fr, tp = timed_form(precision, name)
         others[f'{name}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
@@ -330,7 +338,8 @@ def main():
     if args.cpu_baseline in ('auto', 'full') and not stream:
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
         cpu = cpu_baseline(system, k0, k1, full=args.cpu_baseline == 'full',
-                           gpu_result={'fp64': (terms64, F64), args.form: (terms, F)})
+                           gpu_result={'fp64': (terms64, F64), args.form: (terms, F)} if side
+                           else {args.form: (terms, F)})
     form_text = {'fp64': 'plain FP64',
                  'fp32': 'FP32 form (every contraction on FP32 matrix cores, FP64 storage and reductions)',
                  'mixed': 'mixed form (FP32 matrix cores for the Fock intermediates K~, L, R, Xi, G, X_i/Y_i; FP64 '

@@ -27,7 +27,6 @@ namespace {
 
 constexpr int BK = 16;
 constexpr int NT = 256;
-constexpr int kF32Chunk = 4096;  // longest FP32-accumulated k range (a multiple of BK)
 
 struct Params {
   GemmDesc d;

@@ -42,9 +42,10 @@ void symmetrise_add(const double* F, double* out, int N, int accumulate, hipStre
 void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
 // out = (in diag(sc))^T: in is rows x cols (column-major), out is cols x rows.
 void transpose_scale(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
-// out[m + rows (i + ni b)] = d[m] in[m + si i + sb b] for m < rows, i < ni, b < nb.
+// out[m + rows (i + ni b)] = d[m] cb[b] in[m + si i + sb b] for m < rows, i < ni, b < nb
+// (cb == nullptr: 1).
 void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t si, int64_t sb, const double* d,
-                double* out, hipStream_t s);
+                const double* cb, double* out, hipStream_t s);
 void fill(double* p, int64_t n, double v, hipStream_t s);
 
 }  // namespace uw12g

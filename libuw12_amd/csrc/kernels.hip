@@ -199,12 +199,14 @@ __global__ void scale_cols_kernel(const double* __restrict__ in, int64_t rows, i
 // writes are both contiguous runs of `rows` doubles.
 __global__ __launch_bounds__(256) void scale_rows_kernel(const double* __restrict__ in, int64_t rows, int64_t ni,
                                                          int64_t ncol, int64_t si, int64_t sb,
-                                                         const double* __restrict__ d, double* __restrict__ out) {
+                                                         const double* __restrict__ d, const double* __restrict__ cb,
+                                                         double* __restrict__ out) {
   for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
     const int64_t i = col % ni, b = col / ni;
+    const double c = cb ? cb[b] : 1.0;
     const double* src = in + si * i + sb * b;
     double* dst = out + rows * col;
-    for (int64_t m = threadIdx.x; m < rows; m += blockDim.x) dst[m] = d[m] * src[m];
+    for (int64_t m = threadIdx.x; m < rows; m += blockDim.x) dst[m] = c * d[m] * src[m];
   }
 }
 
@@ -326,11 +328,11 @@ void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, 
 }
 
 void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t si, int64_t sb, const double* d,
-                double* out, hipStream_t s) {
+                const double* cb, double* out, hipStream_t s) {
   const int64_t ncol = ni * nb;
   if (rows <= 0 || ncol <= 0) return;
   const unsigned grid = unsigned(std::min<int64_t>(ncol, 65536));
-  hipLaunchKernelGGL(scale_rows_kernel, dim3(grid), dim3(256), 0, s, in, rows, ni, ncol, si, sb, d, out);
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(grid), dim3(256), 0, s, in, rows, ni, ncol, si, sb, d, cb, out);
   UW12G_HIP(hipGetLastError());
 }
 

@@ -828,7 +828,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     run_gemm(ctx, t, "4e_tab", true);
     // S+ Phi_V[:, c+i, B] = U PhiT_V[:, i, B] with PhiT_V[m, i, B] = (1/s_m) Psi_V[m, c+i, B]
     timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DV, 16.0 * Mr * a * DV, [&] {
-      scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, q.phitV, st);
+      scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, nullptr, q.phitV, st);
     });
     GemmDesc x;  // X[A, B] = sum_{m, i} Psi_W[m, c+i, A] PhiT_V[m, i, B]  (direct_utils.cpp:62-133)
     x.M = DW;
@@ -911,8 +911,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       // t~ dX/dD (direct_utils.cpp:242-347): F -= sum_AB Tb_AB Gamma_W[:, a, A]^T S+ Gamma_V[:, b, B]
       // = sum_{m, A} Gt_W[m, a, A] (1/s_m) Xi[m, b, A], Xi = Gt_V Tb^T over B
       const size_t MrN = size_t(Mr) * N;
-      const int nb = int(std::max<size_t>(1, std::min<size_t>(DW, (size_t(1) << 27) / std::max<size_t>(MrN, 1))));
-      double* Xi = ctx->buf("ttdx_xi", MrN * nb);
+      // A blocks of Xi within half the block cap, of equal size rounded up to a
+      // multiple of 64 (whole MFMA tiles along A)
+      const int64_t nb_max = std::max<int64_t>(1, std::min<int64_t>(DW, int64_t(ctx->block_cap / 2) /
+                                                                            std::max<int64_t>(int64_t(MrN), 1)));
+      const int64_t nblk = (DW + nb_max - 1) / nb_max;
+      int nb = int((DW + nblk - 1) / nblk);
+      if (nb > 64) nb = int(std::min<int64_t>(std::min<int64_t>(DW, nb_max), (nb + 63) / 64 * 64));
+      double* Xi = ctx->buf("ttdx_xi", MrN * std::max(nb, 1));
       for (int A0 = 0; Mr > 0 && A0 < DW; A0 += nb) {
         const int na = std::min(nb, DW - A0);
         GemmDesc a1;  // Xi[(m, beta), A] = sum_B Gt_V[(m, beta), B] Tb[A0 + A, B]
@@ -1117,15 +1123,27 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       }
       // ---- 3e indirect Fock (indirect_utils.cpp:138-345)
       if (calc_fock && Mr > 0) {
-        double* phitW = ctx->buf("phitW", size_t(Mr) * a * DW);
+        // the K~ operands PhiT_Y sY = diag(1/s) Psi_Y[:, act] diag(sY): the DF signs folded in,
+        // so the K~ contraction needs no per-k scale
+        double* phitWs = ctx->buf("phitWs", size_t(Mr) * a * DW);
+        double* phitVs = ctx->buf("phitVs", size_t(Mr) * a * DV);
         timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DW, 16.0 * Mr * a * DW, [&] {
-          scale_rows(q.psiW + size_t(Mr) * c, Mr, a, DW, Mr, int64_t(Mr) * o, P->dinv.p, phitW, st);
+          scale_rows(q.psiW + size_t(Mr) * c, Mr, a, DW, Mr, int64_t(Mr) * o, P->dinv.p, W->sign.p, phitWs, st);
+        });
+        timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DV, 16.0 * Mr * a * DV, [&] {
+          scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, V->sign.p, phitVs, st);
         });
         const size_t MrN = size_t(Mr) * N;
         double* L = ctx->buf("blk2", MrN * std::max(DW, DV));
         const int64_t MNo = int64_t(MrN) * o;
         const size_t cap = ctx->block_cap;
-        const int nj = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1))));
+        // j-block: K~ within the block cap and, when L runs in FP32, its k = o jn within one
+        // FP32 range (no split-K pass over L); blocks of equal size
+        int64_t nj_max = std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1));
+        if (ctx->prec != 0) nj_max = std::min<int64_t>(nj_max, kF32Chunk / std::max(o, 1));
+        nj_max = std::max<int64_t>(nj_max, 1);
+        const int64_t n_blocks = (a + nj_max - 1) / nj_max;
+        const int nj = int((a + n_blocks - 1) / n_blocks);
         double* Kt = ctx->buf("blk1", size_t(MNo) * nj);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
@@ -1139,14 +1157,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
           const uw12g_base_integrals* Y = pass == 0 ? V : W;
-          const double* phitY = pass == 0 ? q.phitV : phitW;
+          const double* phitY = pass == 0 ? phitVs : phitWs;
           const double* phiY = pass == 0 ? q.phiV : q.phiW;
           const int DX = X->D, DY = Y->D;
           timed(ctx, "3e_ind_fock_x2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DX,
                 [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
           for (int j0 = 0; j0 < a; j0 += nj) {
             const int jn = std::min(nj, a - j0);
-            GemmDesc g;  // K~[m, (beta, k) | j] = sum_B PhiT_Y[m, j, B] sY_B Phi_Y[beta, k, B]
+            GemmDesc g;  // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B]
             g.M = Mr;
             g.N = N * o;
             g.K = DY;
@@ -1155,8 +1173,6 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             g.Am = lin(1);
             g.Ak = lin(int64_t(Mr) * a);
             g.Ab = Mr;
-            g.ks = Y->sign.p;
-            g.Km = lin(1);
             g.B = phiY;
             g.Bk = lin(int64_t(N) * o);
             g.Bn = lin(1);

@@ -67,6 +67,10 @@ inline Map2 two(uint32_t n0, int64_t s0, int64_t s1) {
 
 // C(m,n) <- alpha sum_k A(m,k) ks(k) B(k,n) + beta C(m,n), for every batch b.
 // A(m,k) = A[Ab*b + Am.off(m) + Ak.off(k)], and likewise for B, C, ks.
+// Longest k range a reduced-precision (prec 1) contraction accumulates in FP32;
+// longer sums are split into such ranges and the partial results added in FP64.
+constexpr int kF32Chunk = 4096;
+
 struct GemmDesc {
   int M = 0, N = 0, K = 0, batch = 1;
   double alpha = 1.0, beta = 0.0;

@@ -732,7 +732,10 @@ double indirect_3el_energy(const Ints& W, const Ints& V, const ABS& P, int s) {
   return acc;
 }
 
-// fock_i summed over (j, k), indirect_utils.cpp:138-194 with X3 GEMVs batched over j
+// fock_i summed over (j, k), indirect_utils.cpp:138-194 with X3 GEMVs batched over j.
+// The k loop runs in order, its per-k products on all threads (threaded BLAS);
+// the j loop inside runs in parallel.  So every k costs the same however many
+// k are evaluated, which keeps a strided sample proportional to the full sum.
 Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
   const Base &Xb = *X.b, &Yb = *Y.b;
   const int N = Xb.N, Nr = Xb.Nr, NdX = Xb.Nd, NdY = Yb.Nd, PP = N * (N + 1) / 2;
@@ -740,20 +743,20 @@ Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
   const Mat &X2 = X.two[s], &Y1 = Y.one[s], &Yr = Y.one_ri[s];
   const int nth = omp_get_max_threads();
   std::vector<Mat> part(nth, Mat(N, N));
+  Mat xt(NdX, a), xam(Nr * N, a), xap(PP, a), Yk(NdY, N);
   Progress prog("indirect 3e Fock (i, j terms) outer k", long((o + g_stride - 1) / g_stride));
+  for (int k = 0; k < o; k += g_stride) {
+    for (int j = 0; j < a; ++j)
+      for (int A = 0; A < NdX; ++A) xt(A, j) = X2(k * a + j, A) * Xb.vals[A];
+    gemm('N', 'N', Nr * N, a, NdX, 1.0, Xb.J3ri.d.data(), Nr * N, xt.d.data(), NdX, 0.0, xam.d.data(), Nr * N);
+    gemm('N', 'N', PP, a, NdX, 1.0, Xb.J3.d.data(), PP, xt.d.data(), NdX, 0.0, xap.d.data(), PP);
+    for (int b = 0; b < N; ++b)
+      for (int B = 0; B < NdY; ++B) Yk(B, b) = Yb.vals[B] * Y1(k * N + b, B);
 #pragma omp parallel
-  {
-    Mat& f = part[omp_get_thread_num()];
-    Mat xt(NdX, a), xam(Nr * N, a), xap(PP, a), Yk(NdY, N), yvj(Nr, N), ypj(N, N);
-    Mat t1(Nr, N), t2(N, N), sq(N, N);
+    {
+      Mat& f = part[omp_get_thread_num()];
+      Mat yvj(Nr, N), ypj(N, N), t1(Nr, N), t2(N, N), sq(N, N);
 #pragma omp for schedule(dynamic)
-    for (int k = 0; k < o; k += g_stride) {
-      for (int j = 0; j < a; ++j)
-        for (int A = 0; A < NdX; ++A) xt(A, j) = X2(k * a + j, A) * Xb.vals[A];
-      gemm('N', 'N', Nr * N, a, NdX, 1.0, Xb.J3ri.d.data(), Nr * N, xt.d.data(), NdX, 0.0, xam.d.data(), Nr * N);
-      gemm('N', 'N', PP, a, NdX, 1.0, Xb.J3.d.data(), PP, xt.d.data(), NdX, 0.0, xap.d.data(), PP);
-      for (int b = 0; b < N; ++b)
-        for (int B = 0; B < NdY; ++B) Yk(B, b) = Yb.vals[B] * Y1(k * N + b, B);
       for (int j = 0; j < a; ++j) {
         const double* Xam = xam.col(j);  // (Nr x N)
         gemm('N', 'N', Nr, N, NdY, 1.0, Yr.d.data() + (size_t)Nr * j, Yr.r, Yk.d.data(), NdY, 0.0, yvj.d.data(), Nr);
@@ -768,8 +771,8 @@ Mat fock_i_sum(const Ints& X, const Ints& Y, const ABS& P, int s) {
         gemm('N', 'N', N, N, N, 1.0, P.aa.d.data(), N, ypj.d.data(), N, 0.0, t2.d.data(), N);
         gemm('N', 'N', N, N, N, 1.0, sq.d.data(), N, t2.d.data(), N, 1.0, f.d.data(), N);
       }
-      prog.tick();
     }
+    prog.tick();
   }
   Mat F(N, N);
   for (auto& x : part) axpy(F, 1.0, x);
