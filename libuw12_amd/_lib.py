@@ -85,7 +85,7 @@ def load():
                                   vp, Map2, Map2, ctypes.c_longlong,
                                   vp, Map2, Map2, ctypes.c_longlong,
                                   vp, Map2, Map2, ctypes.c_longlong,
-                                  vp, Map2, ctypes.c_int, dp]
+                                  vp, Map2, ctypes.c_int, ctypes.c_int, dp]
     L.uw12g_test_set_stream_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_set_exact_stages.argtypes = [vp, ctypes.c_char_p]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]

@@ -88,10 +88,12 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
 extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta,
                                const double* A, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B,
                                uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn,
-                               long long Cb, const double* ks, uw12g_map2 Km, int reps, double* ms) {
+                               long long Cb, const double* ks, uw12g_map2 Km, int prec, int reps, double* ms) {
   try {
+    if (prec != 0 && prec != 1) throw Error(-1, "prec must be 0 (FP64) or 1 (FP32)");
     UW12G_HIP(hipSetDevice(device));
     GemmDesc d;
+    d.prec = prec;
     d.M = M;
     d.N = N;
     d.K = K;
