@@ -201,12 +201,15 @@ def main():
                     help='precision form of the timed steps (the plain FP64 path is run beside it)')
     ap.add_argument('--side-reps', type=int, default=3,
                     help='timed evaluations of each side form (after one untimed one when more than 1)')
-    ap.add_argument('--side-forms', default='auto', choices=['auto', 'skip'],
-                    help='skip: no FP64 / FP32 / mixed evaluations beside the timed ones (no accuracy gate); '
-                         'for profiler runs that should see only the timed form')
+    ap.add_argument('--side-forms', default='auto', choices=['auto', 'gate', 'skip'],
+                    help='gate: only the FP64 evaluation the accuracy gate needs; skip: no FP64 / FP32 / mixed '
+                         'evaluations beside the timed ones (no gate), for profiler runs that should see only the '
+                         'timed form')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
                     help='auto: sampled oracle timing; full: one complete oracle evaluation, with parity report')
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
+    ap.add_argument('--verbose', action='store_true',
+                    help='library progress lines on stderr during each evaluation (long configurations)')
     ap.add_argument('--calibrate-copy', action='store_true',
                     help='only copy 1 GiB device to device 3 times (reference for HBM byte counters) and exit')
     args = ap.parse_args()
@@ -240,6 +243,8 @@ def main():
     print(f'[bench] {args.config}: n_ao={N} n_df={Nd} n_ri={Nr} n_occ={occ_counts} generated in {t_gen:.1f}s',
           file=sys.stderr, flush=True)
     ctx = u.Context(0)
+    if args.verbose:
+        ctx.set_verbosity(1)
     t = time.perf_counter()
     if stream:
         handles = []
@@ -310,7 +315,7 @@ def main():
         return tr, Fr, tp
 
     # the plain FP64 path on the same inputs: reference for the reduced-precision forms
-    side = args.side_forms == 'auto'
+    side = args.side_forms in ('auto', 'gate')
     if args.form == 'fp64' or not side:
         terms64, F64, t64 = terms, F, elapsed / args.steps
     else:
@@ -329,7 +334,7 @@ def main():
               file=sys.stderr, flush=True)
     others = {}
     for name, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
-        if name == args.form or not side:
+        if name == args.form or args.side_forms != 'auto':
             continue
         tr, Fr, tp = timed_form(precision, name)
         others[f'{name}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),

@@ -146,6 +146,12 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
  * projector handles of the forward call must still exist. */
 int uw12g_adjoint(uw12g_context* ctx, unsigned long long tape, double* fock);
 
+/* Progress lines on stderr at the major stages of every evaluation on this
+ * context when level > 0; each line first waits for the GPU work queued before
+ * it.  Counterpart of the reference's print_level argument of uw12_energy /
+ * uw12_fock (libuw12.hpp:129-140), which prints the energy pieces. */
+int uw12g_context_set_verbosity(uw12g_context* ctx, int level);
+
 /* Kernel-level timing of the most recent evaluation on this context:
  * names[i] / milliseconds[i] for up to cap entries; returns the count.
  * Enabled by uw12g_context_set_profiling(ctx, 1).  No reference counterpart. */

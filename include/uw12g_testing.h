@@ -42,6 +42,11 @@ int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, do
  * this context; small values force many shell groups (tests). */
 int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles);
 
+/* Doubles allowed for one blocked intermediate of the same-spin terms on this
+ * context (K~, X_i / Y_i, GW / GV blocks; L and Xi use twice / half of it);
+ * small values force many blocks (tests).  0 restores the default. */
+int uw12g_test_set_block_cap(uw12g_context* ctx, size_t doubles);
+
 /* Comma-separated stage names whose contractions the mixed form also runs in
  * FP64 on this context (error attribution); NULL or "" clears. */
 int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);

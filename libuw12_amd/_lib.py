@@ -24,9 +24,10 @@ EXPORTS = (
     'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
     'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
+    'uw12g_context_set_verbosity',
     'uw12g_context_profile',
     'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm', 'uw12g_test_set_stream_cap',
-    'uw12g_test_set_exact_stages',
+    'uw12g_test_set_exact_stages', 'uw12g_test_set_block_cap',
 )
 
 
@@ -73,6 +74,7 @@ def load():
                                                 ctypes.POINTER(ctypes.c_ulonglong)]
     L.uw12g_adjoint.argtypes = [vp, ctypes.c_ulonglong, dp]
     L.uw12g_context_set_profiling.argtypes = [vp, ctypes.c_int]
+    L.uw12g_context_set_verbosity.argtypes = [vp, ctypes.c_int]
     L.uw12g_context_profile.argtypes = [vp, ctypes.POINTER(ctypes.c_char_p), dp, dp, dp, ctypes.c_int]
     L.uw12g_test_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_double,
@@ -88,6 +90,7 @@ def load():
                                   vp, Map2, ctypes.c_int, ctypes.c_int, dp]
     L.uw12g_test_set_stream_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_set_exact_stages.argtypes = [vp, ctypes.c_char_p]
+    L.uw12g_test_set_block_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]
     L.uw12g_test_map2.restype = ctypes.c_longlong
     _lib = L

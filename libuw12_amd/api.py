@@ -46,6 +46,10 @@ class Context:
     def set_profiling(self, on=True):
         check(_lib.load().uw12g_context_set_profiling(self._h, 1 if on else 0))
 
+    def set_verbosity(self, level=1):
+        """Progress lines on stderr at the major stages of each evaluation (level > 0)."""
+        check(_lib.load().uw12g_context_set_verbosity(self._h, int(level)))
+
     def profile(self, cap=4096):
         """Per-launch records of the last evaluation: list of (name, ms, flops, bytes)."""
         L = _lib.load()

@@ -163,8 +163,11 @@ struct uw12g_context {
   // doubles allowed for one blocked intermediate of the indirect terms (K~,
   // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
   size_t block_cap = size_t(1) << 28;
+  size_t block_cap_default = size_t(1) << 28;
   int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
   std::vector<std::string> exact_extra;  // stage names also kept FP64 in the mixed form (tests)
+  int verbosity = 0;  // > 0: progress lines on stderr at the major stages of an evaluation
+  std::chrono::steady_clock::time_point t_eval;
   size_t stream_cap = size_t(1) << 25;  // doubles per host staging buffer of the per-shell setup
   // Differentiation form: arguments of the last forward call.  Its
   // intermediates live in the named buffers until the next fused or forward
@@ -525,6 +528,19 @@ double spin_factor(int ns, int s, int t, double k0, double k1) {  // four_electr
   return s == t ? k1 : k0;
 }
 
+// A progress line on stderr once the stream has finished the work queued so
+// far (verbosity > 0 only: the wait serialises the host with the GPU).
+template <class... Args>
+void progress(uw12g_context* ctx, const char* fmt, Args... args) {
+  if (ctx->verbosity <= 0) return;
+  UW12G_HIP(hipStreamSynchronize(ctx->stream));
+  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - ctx->t_eval).count();
+  std::fprintf(stderr, "[uw12g] %8.2fs ", t);
+  std::fprintf(stderr, fmt, args...);
+  std::fprintf(stderr, "\n");
+  std::fflush(stderr);
+}
+
 // U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
 // eigenvectors U of the ABS overlap.  With S+ = U diag(1/s) U^T every
 // contraction over the ABS index runs over those r rows, S+ becoming the
@@ -588,6 +604,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
   const bool do_ind = indirect_term && k1 != 0.0;
   const double e = ns == 1 ? 2.0 : 1.0;
   ctx->prof.clear();
+  ctx->t_eval = std::chrono::steady_clock::now();
   ensure_rotated(ctx, W, P);
   ensure_rotated(ctx, V, P);
 
@@ -992,99 +1009,131 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         timed(ctx, "4e_ind_energy", "e4i_kernel", 2.0 * ao * ao, 16.0 * ao * ao,
               [&] { e4i_energy(W4, V4, a, o, -0.5 * e * k1, Eacc + UW12G_E4_INDIRECT, partial, st); });
       }
-      // ---- 4e indirect Fock (four_electron.cpp:181-249)
+      // ---- 4e indirect Fock (four_electron.cpp:181-249), blocked over i (and l) so that
+      // the X_i / Y_i blocks stay within the block cap
       if (calc_fock) {
-        const size_t nXY = size_t(N) * ao * a;
-        double* Xb = ctx->buf("blk1", nXY);
-        double* Yb = ctx->buf("blk2", nXY);
-        GemmDesc gx;  // X_i[alpha, (j,k)] = sum_A Phi_W[alpha, c+i, A] sW_A O2_W[(j,k), A]
-        gx.M = N;
-        gx.N = int(ao);
-        gx.K = DW;
-        gx.batch = a;
-        gx.A = q.phiW + size_t(N) * c;
-        gx.Am = lin(1);
-        gx.Ak = lin(int64_t(N) * o);
-        gx.Ab = N;
-        gx.B = W2s;
-        gx.Bk = lin(ao);
-        gx.Bn = lin(1);
-        gx.C = Xb;
-        gx.Cm = lin(1);
-        gx.Cn = lin(N);
-        gx.Cb = int64_t(N) * ao;
-        run_gemm(ctx, gx, "4e_ind_fock_x");
         // V2sT[B + DV (i + a k)] = sV_B O2_V[i + a k, B]: the DF index contiguous, so the Y_i
         // contraction reads its B operand along unit stride
         double* V2sT = ctx->buf("V2sT", size_t(ao) * DV);
         timed(ctx, "4e_ind_fock_v2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DV,
               [&] { transpose_scale(q.o2V, ao, DV, V->sign.p, V2sT, st); });
-        GemmDesc gy;  // Y_i[(beta, j), k] = sum_B Phi_V[beta, c+j, B] sV_B O2_V[i + a k, B]
-        gy.M = N * a;
-        gy.N = o;
-        gy.K = DV;
-        gy.batch = a;
-        gy.A = q.phiV + size_t(N) * c;
-        gy.Am = lin(1);
-        gy.Ak = lin(int64_t(N) * o);
-        gy.B = V2sT;
-        gy.Bk = lin(1);
-        gy.Bn = lin(int64_t(DV) * a);
-        gy.Bb = DV;
-        gy.C = Yb;
-        gy.Cm = lin(1);
-        gy.Cn = lin(int64_t(N) * a);
-        gy.Cb = int64_t(N) * ao;
-        run_gemm(ctx, gy, "4e_ind_fock_y");
+        const int64_t per_i = int64_t(N) * ao;
+        const int ni = int(std::max<int64_t>(1, std::min<int64_t>(a, int64_t(ctx->block_cap) / per_i)));
+        double* Xb = ctx->buf("blk1", size_t(per_i) * ni);
+        double* Yb = ctx->buf("blk2", size_t(per_i) * ni);
         const double fz = c > 0 ? 0.5 : 1.0;  // frozen core: 1/2 (F_act + F_frozen)
-        GemmDesc gf;  // F -= 2 k1 sum_{(j,k,i)} X[alpha, .] Y[beta, .]
-        gf.M = N;
-        gf.N = N;
-        gf.K = int(ao * a);
-        gf.alpha = -2.0 * k1 * fz;
-        gf.beta = 1.0;
-        gf.A = Xb;
-        gf.Am = lin(1);
-        gf.Ak = lin(N);
-        gf.B = Yb;
-        gf.Bk = lin(N);
-        gf.Bn = lin(1);
-        gf.C = Fraw + NN * s;
-        gf.Cm = lin(1);
-        gf.Cn = lin(N);
-        run_gemm(ctx, gf, "4e_ind_fock_f", true);
-        if (c > 0) {
-          const size_t nXY2 = size_t(N) * ao * o;
-          double* X2b = ctx->buf("blk3", nXY2);
-          double* Y2b = ctx->buf("blk4", nXY2);
-          GemmDesc hx = gx;  // X'_l for every occupied l
-          hx.batch = o;
-          hx.A = q.phiW;
-          hx.C = X2b;
-          run_gemm(ctx, hx, "4e_ind_fock_x");
-          GemmDesc hy;  // Y'_l[(beta, k), j] = sum_B Phi_V[beta, k, B] sV_B O2_V[j + a l, B]
-          hy.M = N * o;
-          hy.N = a;
-          hy.K = DV;
-          hy.batch = o;
-          hy.A = q.phiV;
-          hy.Am = lin(1);
-          hy.Ak = lin(int64_t(N) * o);
-          hy.B = V2s;
-          hy.Bk = lin(ao);
-          hy.Bn = lin(1);
-          hy.Bb = a;
-          hy.C = Y2b;
-          hy.Cm = two(uint32_t(N), 1, int64_t(N) * a);
-          hy.Cn = lin(N);
-          hy.Cb = int64_t(N) * ao;
-          run_gemm(ctx, hy, "4e_ind_fock_y");
-          GemmDesc hf = gf;
-          hf.K = int(ao * o);
-          hf.A = X2b;
-          hf.B = Y2b;
-          run_gemm(ctx, hf, "4e_ind_fock_f", true);
+        for (int i0 = 0; i0 < a; i0 += ni) {
+          const int in = std::min(ni, a - i0);
+          GemmDesc gx;  // X_i[alpha, (j,k)] = sum_A Phi_W[alpha, c+i, A] sW_A O2_W[(j,k), A]
+          gx.M = N;
+          gx.N = int(ao);
+          gx.K = DW;
+          gx.batch = in;
+          gx.A = q.phiW + size_t(N) * (c + i0);
+          gx.Am = lin(1);
+          gx.Ak = lin(int64_t(N) * o);
+          gx.Ab = N;
+          gx.B = W2s;
+          gx.Bk = lin(ao);
+          gx.Bn = lin(1);
+          gx.C = Xb;
+          gx.Cm = lin(1);
+          gx.Cn = lin(N);
+          gx.Cb = per_i;
+          run_gemm(ctx, gx, "4e_ind_fock_x");
+          GemmDesc gy;  // Y_i[(beta, j), k] = sum_B Phi_V[beta, c+j, B] sV_B O2_V[i + a k, B]
+          gy.M = N * a;
+          gy.N = o;
+          gy.K = DV;
+          gy.batch = in;
+          gy.A = q.phiV + size_t(N) * c;
+          gy.Am = lin(1);
+          gy.Ak = lin(int64_t(N) * o);
+          gy.B = V2sT + int64_t(DV) * i0;
+          gy.Bk = lin(1);
+          gy.Bn = lin(int64_t(DV) * a);
+          gy.Bb = DV;
+          gy.C = Yb;
+          gy.Cm = lin(1);
+          gy.Cn = lin(int64_t(N) * a);
+          gy.Cb = per_i;
+          run_gemm(ctx, gy, "4e_ind_fock_y");
+          GemmDesc gf;  // F -= 2 k1 sum_{(j,k,i)} X[alpha, .] Y[beta, .]
+          gf.M = N;
+          gf.N = N;
+          gf.K = int(ao * in);
+          gf.alpha = -2.0 * k1 * fz;
+          gf.beta = 1.0;
+          gf.A = Xb;
+          gf.Am = lin(1);
+          gf.Ak = lin(N);
+          gf.B = Yb;
+          gf.Bk = lin(N);
+          gf.Bn = lin(1);
+          gf.C = Fraw + NN * s;
+          gf.Cm = lin(1);
+          gf.Cn = lin(N);
+          run_gemm(ctx, gf, "4e_ind_fock_f", true);
         }
+        if (c > 0) {
+          const int nl = int(std::max<int64_t>(1, std::min<int64_t>(o, int64_t(ctx->block_cap) / per_i)));
+          double* X2b = ctx->buf("blk3", size_t(per_i) * nl);
+          double* Y2b = ctx->buf("blk4", size_t(per_i) * nl);
+          for (int l0 = 0; l0 < o; l0 += nl) {
+            const int ln = std::min(nl, o - l0);
+            GemmDesc hx;  // X'_l[alpha, (j,k)] for every occupied l
+            hx.M = N;
+            hx.N = int(ao);
+            hx.K = DW;
+            hx.batch = ln;
+            hx.A = q.phiW + size_t(N) * l0;
+            hx.Am = lin(1);
+            hx.Ak = lin(int64_t(N) * o);
+            hx.Ab = N;
+            hx.B = W2s;
+            hx.Bk = lin(ao);
+            hx.Bn = lin(1);
+            hx.C = X2b;
+            hx.Cm = lin(1);
+            hx.Cn = lin(N);
+            hx.Cb = per_i;
+            run_gemm(ctx, hx, "4e_ind_fock_x");
+            GemmDesc hy;  // Y'_l[(beta, k), j] = sum_B Phi_V[beta, k, B] sV_B O2_V[j + a l, B]
+            hy.M = N * o;
+            hy.N = a;
+            hy.K = DV;
+            hy.batch = ln;
+            hy.A = q.phiV;
+            hy.Am = lin(1);
+            hy.Ak = lin(int64_t(N) * o);
+            hy.B = V2s + int64_t(a) * l0;
+            hy.Bk = lin(ao);
+            hy.Bn = lin(1);
+            hy.Bb = a;
+            hy.C = Y2b;
+            hy.Cm = two(uint32_t(N), 1, int64_t(N) * a);
+            hy.Cn = lin(N);
+            hy.Cb = per_i;
+            run_gemm(ctx, hy, "4e_ind_fock_y");
+            GemmDesc hf;
+            hf.M = N;
+            hf.N = N;
+            hf.K = int(ao * ln);
+            hf.alpha = -2.0 * k1 * fz;
+            hf.beta = 1.0;
+            hf.A = X2b;
+            hf.Am = lin(1);
+            hf.Ak = lin(N);
+            hf.B = Y2b;
+            hf.Bk = lin(N);
+            hf.Bn = lin(1);
+            hf.C = Fraw + NN * s;
+            hf.Cm = lin(1);
+            hf.Cn = lin(N);
+            run_gemm(ctx, hf, "4e_ind_fock_f", true);
+          }
+        }
+        progress(ctx, "4e indirect Fock, spin %d", s);
       }
       // ---- 3e indirect energy (indirect_utils.cpp:19-133), blocked over k.  With the
       // Fock it comes from the Fock's L intermediate instead (below).
@@ -1134,9 +1183,12 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, V->sign.p, phitVs, st);
         });
         const size_t MrN = size_t(Mr) * N;
-        double* L = ctx->buf("blk2", MrN * std::max(DW, DV));
         const int64_t MNo = int64_t(MrN) * o;
         const size_t cap = ctx->block_cap;
+        // L columns (DF index A) in blocks within twice the block cap; each block re-forms K~
+        const int DXmax = std::max(DW, DV);
+        const int64_t nA_max = std::max<int64_t>(1, std::min<int64_t>(DXmax, int64_t(2 * cap) / int64_t(MrN)));
+        double* L = ctx->buf("blk2", MrN * size_t(nA_max));
         // j-block: K~ within the block cap and, when L runs in FP32, its k = o jn within one
         // FP32 range (no split-K pass over L); blocks of equal size
         int64_t nj_max = std::min<int64_t>(a, int64_t(cap) / std::max<int64_t>(MNo, 1));
@@ -1147,12 +1199,13 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         double* Kt = ctx->buf("blk1", size_t(MNo) * nj);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
-        double* X2sT = ctx->buf("blk3", size_t(ao) * std::max(DW, DV));
+        double* X2sT = ctx->buf("blk3", size_t(ao) * DXmax);
         // fock_k's R needs Q~[m, i, alpha | j] = sum_B PhiT_V[m, j, B] sV_B Phi_V[alpha, c+i, B]
         // (indirect_utils.cpp:199-259), which is the first pass's K~[m, (alpha, c+i) | j]: R and
         // fock_k are formed from each K~ block of that pass before it is overwritten.  The ABS
         // index runs over the Mr rotated rows m throughout.
         double* R = ctx->buf("blk4", size_t(DW) * N * nj);
+        double* Lc = ctx->buf("blk5", size_t(Mr) * a * std::min<int64_t>(DW, nA_max));
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
@@ -1162,125 +1215,131 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           const int DX = X->D, DY = Y->D;
           timed(ctx, "3e_ind_fock_x2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DX,
                 [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
-          for (int j0 = 0; j0 < a; j0 += nj) {
-            const int jn = std::min(nj, a - j0);
-            GemmDesc g;  // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B]
-            g.M = Mr;
-            g.N = N * o;
-            g.K = DY;
-            g.batch = jn;
-            g.A = phitY + size_t(Mr) * j0;
-            g.Am = lin(1);
-            g.Ak = lin(int64_t(Mr) * a);
-            g.Ab = Mr;
-            g.B = phiY;
-            g.Bk = lin(int64_t(N) * o);
-            g.Bn = lin(1);
-            g.C = Kt;
-            g.Cm = lin(1);
-            g.Cn = lin(Mr);
-            g.Cb = MNo;
-            run_gemm(ctx, g, "3e_ind_fock_kt");
-            GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
-            h.M = int(MrN);
-            h.N = DX;
-            h.K = o * jn;
-            h.beta = j0 == 0 ? 0.0 : 1.0;
-            h.A = Kt;
-            h.Am = lin(1);
-            h.Ak = lin(int64_t(MrN));
-            h.B = X2sT + int64_t(DX) * j0;
-            h.Bk = two(uint32_t(o), int64_t(DX) * a, DX);
-            h.Bn = lin(1);
-            h.C = L;
-            h.Cm = lin(1);
-            h.Cn = lin(int64_t(MrN));
-            run_gemm(ctx, h, "3e_ind_fock_l");
-            if (pass == 0) {
-              GemmDesc r;  // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j]
-              r.M = DW;
-              r.N = N;
-              r.K = Mr * a;
-              r.batch = jn;
-              r.A = q.psiW + size_t(Mr) * c;
-              r.Am = lin(int64_t(Mr) * o);
-              r.Ak = lin(1);
-              r.B = Kt + MrN * size_t(c);
-              r.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
-              r.Bn = lin(Mr);
-              r.Bb = MNo;
-              r.C = R;
-              r.Cm = lin(1);
-              r.Cn = lin(DW);
-              r.Cb = int64_t(DW) * N;
-              run_gemm(ctx, r, "3e_ind_fock_r");
-              GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
-              f.M = N;
-              f.N = N;
-              f.K = DW * jn;
-              f.alpha = k1;
-              f.beta = 1.0;
-              f.A = q.phiW + size_t(N) * (c + j0);
-              f.Am = lin(1);
-              f.Ak = two(uint32_t(DW), int64_t(N) * o, N);
-              f.ks = W->sign.p;
-              f.Km = two(uint32_t(DW), 1, 0);
-              f.B = R;
-              f.Bk = two(uint32_t(DW), 1, int64_t(DW) * N);
-              f.Bn = lin(DW);
-              f.C = Fraw + NN * s;
-              f.Cm = lin(1);
-              f.Cn = lin(N);
-              run_gemm(ctx, f, "3e_ind_fock_k", true);
+          const int64_t nAb = (DX + nA_max - 1) / nA_max;
+          const int nA = int((DX + nAb - 1) / nAb);
+          for (int A0 = 0; A0 < DX; A0 += nA) {
+            const int An = std::min(nA, DX - A0);
+            for (int j0 = 0; j0 < a; j0 += nj) {
+              const int jn = std::min(nj, a - j0);
+              GemmDesc g;  // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B]
+              g.M = Mr;
+              g.N = N * o;
+              g.K = DY;
+              g.batch = jn;
+              g.A = phitY + size_t(Mr) * j0;
+              g.Am = lin(1);
+              g.Ak = lin(int64_t(Mr) * a);
+              g.Ab = Mr;
+              g.B = phiY;
+              g.Bk = lin(int64_t(N) * o);
+              g.Bn = lin(1);
+              g.C = Kt;
+              g.Cm = lin(1);
+              g.Cn = lin(Mr);
+              g.Cb = MNo;
+              run_gemm(ctx, g, "3e_ind_fock_kt");
+              GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
+              h.M = int(MrN);
+              h.N = An;
+              h.K = o * jn;
+              h.beta = j0 == 0 ? 0.0 : 1.0;
+              h.A = Kt;
+              h.Am = lin(1);
+              h.Ak = lin(int64_t(MrN));
+              h.B = X2sT + int64_t(DX) * j0 + A0;
+              h.Bk = two(uint32_t(o), int64_t(DX) * a, DX);
+              h.Bn = lin(1);
+              h.C = L;
+              h.Cm = lin(1);
+              h.Cn = lin(int64_t(MrN));
+              run_gemm(ctx, h, "3e_ind_fock_l");
+              if (pass == 0 && A0 == 0) {
+                GemmDesc r;  // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j]
+                r.M = DW;
+                r.N = N;
+                r.K = Mr * a;
+                r.batch = jn;
+                r.A = q.psiW + size_t(Mr) * c;
+                r.Am = lin(int64_t(Mr) * o);
+                r.Ak = lin(1);
+                r.B = Kt + MrN * size_t(c);
+                r.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
+                r.Bn = lin(Mr);
+                r.Bb = MNo;
+                r.C = R;
+                r.Cm = lin(1);
+                r.Cn = lin(DW);
+                r.Cb = int64_t(DW) * N;
+                run_gemm(ctx, r, "3e_ind_fock_r");
+                GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
+                f.M = N;
+                f.N = N;
+                f.K = DW * jn;
+                f.alpha = k1;
+                f.beta = 1.0;
+                f.A = q.phiW + size_t(N) * (c + j0);
+                f.Am = lin(1);
+                f.Ak = two(uint32_t(DW), int64_t(N) * o, N);
+                f.ks = W->sign.p;
+                f.Km = two(uint32_t(DW), 1, 0);
+                f.B = R;
+                f.Bk = two(uint32_t(DW), 1, int64_t(DW) * N);
+                f.Bn = lin(DW);
+                f.C = Fraw + NN * s;
+                f.Cm = lin(1);
+                f.Cn = lin(N);
+                run_gemm(ctx, f, "3e_ind_fock_k", true);
+              }
+              progress(ctx, "same-spin 3e Fock, spin %d pass %d, A %d-%d of %d, j %d-%d of %d", s, pass, A0,
+                       A0 + An, DX, j0, j0 + jn, a);
             }
+            if (pass == 0 && mode != kAdjoint) {
+              // E3_ind = e k1 sum_{m, i, A} Psi_W[m, c+i, A] sum_beta L[m, beta, A] Ca[beta, i]:
+              // L holds sum_{j,k,B} PhiT_V[m, j, B] sV_B Phi_V[beta, k, B] W2s[(j, k), A], and Ca^T
+              // over beta turns sV_B Phi_V[beta, k, B] into V2s[(i, k), B] (indirect_utils.cpp:19-133)
+              GemmDesc g;
+              g.M = Mr;
+              g.N = a;
+              g.K = N;
+              g.batch = An;
+              g.A = L;
+              g.Am = lin(1);
+              g.Ak = lin(Mr);
+              g.Ab = int64_t(MrN);
+              g.B = q.Co + size_t(N) * c;
+              g.Bk = lin(1);
+              g.Bn = lin(N);
+              g.C = Lc;
+              g.Cm = lin(1);
+              g.Cn = lin(Mr);
+              g.Cb = int64_t(Mr) * a;
+              run_gemm(ctx, g, "3e_ind_e_lc", true);
+              DotDesc dd;
+              dd.x = q.psiW + size_t(Mr) * c + size_t(Mr) * o * A0;
+              dd.y = Lc;
+              dd.n = int64_t(Mr) * a * An;
+              dd.ld = int64_t(Mr) * a;
+              dd.xs = int64_t(Mr) * o;
+              dd.ys = int64_t(Mr) * a;
+              dot(dd, e * k1, Eacc + UW12G_E3_INDIRECT, 1, partial, st);
+            }
+            GemmDesc f;  // F += k1 sum_{m, A} Gt_X[m, alpha, A] L[m, beta, A]
+            f.M = N;
+            f.N = N;
+            f.K = Mr * An;
+            f.alpha = k1;
+            f.beta = 1.0;
+            f.A = X->gt.p + MrN * A0;
+            f.Am = lin(Mr);
+            f.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
+            f.B = L;
+            f.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
+            f.Bn = lin(Mr);
+            f.C = Fraw + NN * s;
+            f.Cm = lin(1);
+            f.Cn = lin(N);
+            run_gemm(ctx, f, "3e_ind_fock_f", true);
           }
-          if (pass == 0 && mode != kAdjoint) {
-            // E3_ind = e k1 sum_{m, i, A} Psi_W[m, c+i, A] sum_beta L[m, beta, A] Ca[beta, i]:
-            // L holds sum_{j,k,B} PhiT_V[m, j, B] sV_B Phi_V[beta, k, B] W2s[(j, k), A], and Ca^T
-            // over beta turns sV_B Phi_V[beta, k, B] into V2s[(i, k), B] (indirect_utils.cpp:19-133)
-            double* Lc = ctx->buf("blk5", size_t(Mr) * a * DW);
-            GemmDesc g;
-            g.M = Mr;
-            g.N = a;
-            g.K = N;
-            g.batch = DW;
-            g.A = L;
-            g.Am = lin(1);
-            g.Ak = lin(Mr);
-            g.Ab = int64_t(MrN);
-            g.B = q.Co + size_t(N) * c;
-            g.Bk = lin(1);
-            g.Bn = lin(N);
-            g.C = Lc;
-            g.Cm = lin(1);
-            g.Cn = lin(Mr);
-            g.Cb = int64_t(Mr) * a;
-            run_gemm(ctx, g, "3e_ind_e_lc", true);
-            DotDesc dd;
-            dd.x = q.psiW + size_t(Mr) * c;
-            dd.y = Lc;
-            dd.n = int64_t(Mr) * a * DW;
-            dd.ld = int64_t(Mr) * a;
-            dd.xs = int64_t(Mr) * o;
-            dd.ys = int64_t(Mr) * a;
-            dot(dd, e * k1, Eacc + UW12G_E3_INDIRECT, 1, partial, st);
-          }
-          GemmDesc f;  // F += k1 sum_{m, A} Gt_X[m, alpha, A] L[m, beta, A]
-          f.M = N;
-          f.N = N;
-          f.K = Mr * DX;
-          f.alpha = k1;
-          f.beta = 1.0;
-          f.A = X->gt.p;
-          f.Am = lin(Mr);
-          f.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
-          f.B = L;
-          f.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
-          f.Bn = lin(Mr);
-          f.C = Fraw + NN * s;
-          f.Cm = lin(1);
-          f.Cn = lin(N);
-          run_gemm(ctx, f, "3e_ind_fock_f", true);
         }
       }
     }
@@ -1323,6 +1382,7 @@ int uw12g_context_create(int device, uw12g_context** out) {
     size_t free_b = 0, total_b = 0;
     UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
     c->block_cap = std::max<size_t>(size_t(1) << 24, std::min<size_t>(total_b / 16, size_t(16) << 30) / 8);
+    c->block_cap_default = c->block_cap;
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Error(-3, "rocblas_create_handle failed");
     rocblas_set_stream(c->rb, c->stream);
     *out = c.release();
@@ -1587,6 +1647,13 @@ int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles) {
   });
 }
 
+int uw12g_test_set_block_cap(uw12g_context* ctx, size_t doubles) {
+  return guard([&] {
+    require(ctx != nullptr, "null context");
+    ctx->block_cap = doubles > 0 ? doubles : ctx->block_cap_default;
+  });
+}
+
 int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes) {
   return guard([&] {
     require(ctx != nullptr, "null context");
@@ -1601,6 +1668,13 @@ int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes) {
         cur += *c;
       }
     }
+  });
+}
+
+int uw12g_context_set_verbosity(uw12g_context* ctx, int level) {
+  return guard([&] {
+    require(ctx != nullptr, "null context");
+    ctx->verbosity = level;
   });
 }
 
