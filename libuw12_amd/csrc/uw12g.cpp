@@ -22,6 +22,7 @@
 #include <atomic>
 #include <cfloat>
 #include <chrono>
+#include <cstdio>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -535,7 +536,9 @@ void progress(uw12g_context* ctx, const char* fmt, Args... args) {
   if (ctx->verbosity <= 0) return;
   UW12G_HIP(hipStreamSynchronize(ctx->stream));
   const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - ctx->t_eval).count();
-  std::fprintf(stderr, "[uw12g] %8.2fs ", t);
+  size_t free_b = 0, total_b = 0;
+  (void)hipMemGetInfo(&free_b, &total_b);
+  std::fprintf(stderr, "[uw12g] %8.2fs %6.1f GB free  ", t, free_b / 1e9);
   std::fprintf(stderr, fmt, args...);
   std::fprintf(stderr, "\n");
   std::fflush(stderr);
@@ -767,26 +770,50 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     }
   }
 
+  // Psi[m, k, A] = sum_alpha (U^T Gamma)[m, alpha, A] Co[alpha, k] for one spin and operator
+  auto psi_transform = [&](const SpinData& q, int op) {
+    const uw12g_base_integrals* X = op == 0 ? W : V;
+    GemmDesc gp;
+    gp.M = Mr;
+    gp.N = q.o;
+    gp.K = N;
+    gp.batch = X->D;
+    gp.A = X->gt.p;
+    gp.Am = lin(1);
+    gp.Ak = lin(Mr);
+    gp.Ab = int64_t(Mr) * N;
+    gp.B = q.Co;
+    gp.Bk = lin(1);
+    gp.Bn = lin(N);
+    gp.C = op == 0 ? q.psiW : q.psiV;
+    gp.Cm = lin(1);
+    gp.Cn = lin(Mr);
+    gp.Cb = int64_t(Mr) * q.o;
+    run_gemm(ctx, gp, "psi_transform", true);
+  };
+  // A fused evaluation of two spins keeps one set of the large ABS-row buffers (Psi, PhiT_V)
+  // and re-forms Psi per spin for the same-spin terms; forward / adjoint keep both (the tape).
+  const bool share = mode == kFused && ns == 2;
+
   // ================= W/V transforms: Phi, O2, t, PhiT_V, X =================
   for (int s = 0; s < ns; ++s) {
     SpinData& q = sd[s];
     const int o = q.o, a = q.a, c = q.c;
-    const std::string sfx = std::to_string(s);
+    const std::string sfx = std::to_string(s), psfx = share ? std::string("s") : sfx;
     q.phiW = ctx->buf("phiW" + sfx, size_t(N) * o * DW);
     q.phiV = ctx->buf("phiV" + sfx, size_t(N) * o * DV);
-    q.psiW = ctx->buf("psiW" + sfx, size_t(Mr) * o * DW);
-    q.psiV = ctx->buf("psiV" + sfx, size_t(Mr) * o * DV);
+    q.psiW = ctx->buf("psiW" + psfx, size_t(Mr) * o * DW);
+    q.psiV = ctx->buf("psiV" + psfx, size_t(Mr) * o * DV);
     q.o2W = ctx->buf("o2W" + sfx, size_t(a) * o * DW);
     q.o2V = ctx->buf("o2V" + sfx, size_t(a) * o * DV);
     q.t = ctx->buf("t" + sfx, size_t(DW) * DV);
     q.x = ctx->buf("x" + sfx, size_t(DW) * DV);
-    q.phitV = ctx->buf("phitV" + sfx, size_t(Mr) * a * DV);
+    q.phitV = ctx->buf("phitV" + psfx, size_t(Mr) * a * DV);
     if (mode == kAdjoint) continue;  // kept from the forward call
     const double* Ca = q.Co + size_t(N) * c;
     for (int op = 0; op < 2; ++op) {
       const uw12g_base_integrals* X = op == 0 ? W : V;
       double* phi = op == 0 ? q.phiW : q.phiV;
-      double* psi = op == 0 ? q.psiW : q.psiV;
       double* o2 = op == 0 ? q.o2W : q.o2V;
       GemmDesc g;  // Phi[mu, k, A] = sum_alpha Gamma[mu, alpha, A] Co[alpha, k] (AO rows mu)
       g.M = N;
@@ -805,15 +832,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       g.Cn = lin(N);
       g.Cb = int64_t(N) * o;
       run_gemm(ctx, g, "phi_transform", true);
-      GemmDesc gp = g;  // Psi[m, k, A] = sum_alpha (U^T Gamma)[m, alpha, A] Co[alpha, k]
-      gp.M = Mr;
-      gp.A = X->gt.p;
-      gp.Ak = lin(Mr);
-      gp.Ab = int64_t(Mr) * N;
-      gp.C = psi;
-      gp.Cn = lin(Mr);
-      gp.Cb = int64_t(Mr) * o;
-      run_gemm(ctx, gp, "psi_transform", true);
+      psi_transform(q, op);
       GemmDesc h;  // O2[i, (k, A)] = sum_mu Ca[mu, i] Phi[mu, (k, A)]
       h.M = a;
       h.N = o * X->D;
@@ -862,6 +881,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     x.Cn = lin(DW);
     run_gemm(ctx, x, "3e_xab", true);
   }
+  progress(ctx, "orbital transforms, t and X");
   // direct energies (four_electron.cpp:118-119, direct_utils.cpp:136-157)
   for (int s = 0; s < ns; ++s)
     for (int u = 0; u < ns; ++u) {
@@ -935,7 +955,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       const int64_t nblk = (DW + nb_max - 1) / nb_max;
       int nb = int((DW + nblk - 1) / nblk);
       if (nb > 64) nb = int(std::min<int64_t>(std::min<int64_t>(DW, nb_max), (nb + 63) / 64 * 64));
-      double* Xi = ctx->buf("ttdx_xi", MrN * std::max(nb, 1));
+      double* Xi = ctx->buf("blk3", MrN * std::max(nb, 1));
       for (int A0 = 0; Mr > 0 && A0 < DW; A0 += nb) {
         const int na = std::min(nb, DW - A0);
         GemmDesc a1;  // Xi[(m, beta), A] = sum_B Gt_V[(m, beta), B] Tb[A0 + A, B]
@@ -974,6 +994,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     }
   }
 
+  if (calc_fock) progress(ctx, "direct Fock");
   // ================= same-spin (indirect) terms, x k1 =================
   if (do_ind) {
     for (int s = 0; s < ns; ++s) {
@@ -981,6 +1002,12 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
       const int o = q.o, a = q.a, c = q.c;
       if (a == 0 || o == 0) continue;
       const int64_t ao = int64_t(a) * o;
+      if (share) {  // the shared ABS-row buffers hold the other spin's transforms
+        psi_transform(q, 0);
+        psi_transform(q, 1);
+        if (!calc_fock)
+          scale_rows(q.psiV + size_t(Mr) * c, Mr, a, DV, Mr, int64_t(Mr) * o, P->dinv.p, nullptr, q.phitV, st);
+      }
       // signed two-index transforms  W2s = sW o O2_W, V2s = sV o O2_V
       double* W2s = ctx->buf("W2s", size_t(ao) * DW);
       double* V2s = ctx->buf("V2s", size_t(ao) * DV);
@@ -1175,7 +1202,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         // the K~ operands PhiT_Y sY = diag(1/s) Psi_Y[:, act] diag(sY): the DF signs folded in,
         // so the K~ contraction needs no per-k scale
         double* phitWs = ctx->buf("phitWs", size_t(Mr) * a * DW);
-        double* phitVs = ctx->buf("phitVs", size_t(Mr) * a * DV);
+        // a fused evaluation no longer needs the unsigned PhiT_V (X is formed): reuse its buffer
+        double* phitVs = mode == kFused ? q.phitV : ctx->buf("phitVs", size_t(Mr) * a * DV);
         timed(ctx, "3e_abs_apply", "scale_rows_kernel", double(Mr) * a * DW, 16.0 * Mr * a * DW, [&] {
           scale_rows(q.psiW + size_t(Mr) * c, Mr, a, DW, Mr, int64_t(Mr) * o, P->dinv.p, W->sign.p, phitWs, st);
         });
