@@ -61,10 +61,15 @@ def main():
 
     F_adj = {}
     for name, prec in (('fp64', u.FP64), ('mixed', u.MIXED)):
+        t = time.perf_counter()
         _, tape = u.forward(W, V, WV, P, orbs, occ, act, **kw, precision=prec)
+        print(f'[fd_check] {name} forward {time.perf_counter() - t:.1f}s', file=sys.stderr, flush=True)
+        t = time.perf_counter()
         F_adj[name] = u.adjoint(tape)
+        print(f'[fd_check] {name} adjoint {time.perf_counter() - t:.1f}s', file=sys.stderr, flush=True)
+    t = time.perf_counter()
     _, F_fused = u.form_fock(W, V, WV, P, orbs, occ, act, **kw, precision=u.FP64)
-    print('[fd_check] adjoint and fused Fock done', file=sys.stderr, flush=True)
+    print(f'[fd_check] fused FP64 Fock {time.perf_counter() - t:.1f}s', file=sys.stderr, flush=True)
 
     Sao = s['S'][:N, :N]
     rng = np.random.default_rng(4004)
@@ -82,6 +87,7 @@ def main():
             Cs = [np.asfortranarray(c + tt * x) for c, x in zip(orbs, dC)]
             e, _ = u.form_fock(W, V, WV, P, Cs, occ, act, calculate_fock=False, **kw, precision=u.FP64)
             E[tt] = e['total']
+            print(f'[fd_check] direction {d + 1}: energy at t = {tt:+.0f}', file=sys.stderr, flush=True)
         d1 = 0.5 * (E[1.0] - E[-1.0])
         d2 = 0.25 * (E[2.0] - E[-2.0])
         fd = (4.0 * d1 - d2) / 3.0
