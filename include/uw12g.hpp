@@ -131,6 +131,17 @@ inline void check_channels(const Orbitals& orb, const Occupations& occ, const st
   if (n_active.size() != orb.n_spin)
     throw std::runtime_error("Different number of spin channels in n_active and orbitals");
 }
+
+// print_level > 1: the library's progress lines for the duration of one call
+struct ProgressScope {
+  bool on;
+  explicit ProgressScope(size_t print_level) : on(print_level > 1) {
+    if (on) check(uw12g_context_set_verbosity(context(), 1));
+  }
+  ~ProgressScope() {
+    if (on) (void)uw12g_context_set_verbosity(context(), 0);
+  }
+};
 }  // namespace detail
 
 inline double uw12_energy(const BaseIntegrals& W, const BaseIntegrals& V, const BaseIntegrals& WV,
@@ -138,6 +149,7 @@ inline double uw12_energy(const BaseIntegrals& W, const BaseIntegrals& V, const 
                           const std::vector<size_t>& n_active, double scale_opp_spin, double scale_same_spin,
                           size_t print_level = 0) {
   detail::check_channels(orbitals, occ, n_active);
+  detail::ProgressScope progress(print_level);
   double e = 0.0;
   detail::check(uw12g_energy(detail::context(), W.handle(), V.handle(), WV.handle(), abs_projectors.handle(),
                              orbitals.C.data(), orbitals.n_ao, orbitals.n_orb, orbitals.n_spin, occ.occ.data(),
@@ -151,6 +163,7 @@ inline double uw12_fock(double* fock, const BaseIntegrals& W, const BaseIntegral
                         const std::vector<size_t>& n_active, double scale_opp_spin, double scale_same_spin,
                         size_t print_level = 0) {
   detail::check_channels(orbitals, occ, n_active);
+  detail::ProgressScope progress(print_level);
   double e = 0.0;
   detail::check(uw12g_fock(detail::context(), fock, W.handle(), V.handle(), WV.handle(), abs_projectors.handle(),
                            orbitals.C.data(), orbitals.n_ao, orbitals.n_orb, orbitals.n_spin, occ.occ.data(),
