@@ -51,6 +51,11 @@ int uw12g_test_set_block_cap(uw12g_context* ctx, size_t doubles);
  * FP64 on this context (error attribution); NULL or "" clears. */
 int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);
 
+/* Force the GEMM tile shape (64 or 128 rows x 64 or 128 columns) for every
+ * later launch in the process; 0, 0 restores the automatic choice (tests and
+ * tile-choice measurements). */
+int uw12g_test_force_tile(int bm, int bn);
+
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);
 

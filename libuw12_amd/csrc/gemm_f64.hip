@@ -23,6 +23,8 @@ namespace uw12g {
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
+int g_force_tile[2] = {0, 0};
+
 namespace {
 
 constexpr int BK = 16;
@@ -562,13 +564,20 @@ namespace {
 void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hipStream_t stream, Workspace* ws,
                  Profiler* prof) {
   // Tile shape: the best product of padding efficiency (useful fraction of
-  // the padded output) and the relative speed of the tile shape.
+  // the padded output) and the relative speed of the tile shape.  Small
+  // outputs (at most 64 tiles of 64 x 64, e.g. the N x N Fock sums over long
+  // k, which run split-K) lose little with small tiles: measured relative
+  // speeds there are ~0.96 / 0.86 for the half / quarter tiles, against ~0.9 /
+  // 0.7 on large outputs.
+  const bool small = int64_t((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch <= 64;
   int BM = 128, BN = 128;
   double best = -1.0;
   for (int bm : {128, 64})
     for (int bn : {128, 64}) {
+      if (g_force_tile[0] && (bm != g_force_tile[0] || bn != g_force_tile[1])) continue;
       const double padded = double((d.M + bm - 1) / bm * bm) * double((d.N + bn - 1) / bn * bn);
-      const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? 0.7 : 0.9;
+      const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? (small ? 0.9 : 0.7)
+                                                                                    : (small ? 0.95 : 0.9);
       const double score = speed * double(d.M) * double(d.N) / padded;
       if (score > best + 1e-12) {
         best = score;

@@ -33,6 +33,17 @@ struct Dev {
 
 extern "C" long long uw12g_test_map2(uw12g_map2 m, unsigned i) { return (long long)to_map(m).off(i); }
 
+extern "C" int uw12g_test_force_tile(int bm, int bn) {
+  const bool ok = (bm == 0 && bn == 0) || ((bm == 64 || bm == 128) && (bn == 64 || bn == 128));
+  if (!ok) {
+    set_last_error("tile must be 0 x 0 (automatic) or 64 / 128 in each dimension");
+    return -1;
+  }
+  g_force_tile[0] = bm;
+  g_force_tile[1] = bn;
+  return 0;
+}
+
 extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, double beta,
                                const double* A, size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab,
                                const double* B, size_t nB, uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C,

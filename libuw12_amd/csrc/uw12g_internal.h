@@ -118,6 +118,8 @@ struct Profiler {
 
 struct Workspace;
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof = nullptr);
+// Test hook: a nonzero {BM, BN} forces that tile shape for every launch in the process.
+extern int g_force_tile[2];
 void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
 
 // Device scratch owned by a context: grown on demand, reused across calls.
