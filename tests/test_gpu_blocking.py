@@ -43,3 +43,23 @@ def test_blocked_equals_unblocked(gpu_ctx, name, n_active, cap, precision):
         assert abs(q1[k] - q0[k]) <= rel * max(abs(q0[k]), 1e-3 * scale), (k, q1[k], q0[k])
     fs = max(float(np.abs(f).max()) for f in F0)
     assert fock_error(F1, F0) <= (1e-10 if precision == u.FP64 else 2e-6) * fs
+
+
+def test_progress_lines(gpu_ctx, capfd):
+    """uw12g_context_set_verbosity: one line per major stage on stderr, the results unchanged."""
+    print('case c1 progress lines', flush=True)
+    s = synth.make_system('c1')
+    h = gpu_setup(s, gpu_ctx)
+    e0, F0 = u.form_fock(*h, s['orbitals'], s['occ'], s['n_active'], scale_same_spin=0.5)
+    capfd.readouterr()
+    gpu_ctx.set_verbosity(1)
+    try:
+        e1, F1 = u.form_fock(*h, s['orbitals'], s['occ'], s['n_active'], scale_same_spin=0.5)
+    finally:
+        gpu_ctx.set_verbosity(0)
+    err = capfd.readouterr().err
+    for stage in ('orbital transforms', 'direct Fock', '4e indirect Fock', 'same-spin 3e Fock'):
+        assert f'{stage}' in err, err
+    assert all(line.startswith('[uw12g]') for line in err.splitlines() if 'uw12g' in line)
+    assert all(e0[k] == e1[k] for k in TERMS)
+    assert all(np.array_equal(a, b) for a, b in zip(F0, F1))
