@@ -46,6 +46,10 @@ void transpose_scale(const double* in, int64_t rows, int64_t cols, const double*
 // (cb == nullptr: 1).
 void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t si, int64_t sb, const double* d,
                 const double* cb, double* out, hipStream_t s);
+// Columns of one block of occupied pairs (j, k), j in [j0, j0 + nj) active, k in
+// [0, c) (core) and [c + j0, c + a) (active from j0 on): out[r + rows (kk + nk jj)]
+// = in[r + rows (j + a k)] with j = j0 + jj, k = kk (kk < c) or kk + j0, nk = c + a - j0.
+void pair_gather(const double* in, int64_t rows, int a, int c, int j0, int nj, int nk, double* out, hipStream_t s);
 void fill(double* p, int64_t n, double v, hipStream_t s);
 
 }  // namespace uw12g

@@ -210,6 +210,20 @@ __global__ __launch_bounds__(256) void scale_rows_kernel(const double* __restric
   }
 }
 
+// Column q = kk + nk jj of one block of occupied pairs: j = j0 + jj, k = kk for
+// kk < c (core), else kk + j0; out[r + rows q] = in[r + rows (j + a k)].  One
+// block per column, the rows along the threads.
+__global__ __launch_bounds__(256) void pair_gather_kernel(const double* __restrict__ in, int64_t rows, int a, int c,
+                                                          int j0, int nk, int64_t ncol, double* __restrict__ out) {
+  for (int64_t q = blockIdx.x; q < ncol; q += gridDim.x) {
+    const int kk = int(q % nk), jj = int(q / nk);
+    const int k = kk < c ? kk : kk + j0;
+    const double* src = in + rows * (int64_t(j0 + jj) + int64_t(a) * k);
+    double* dst = out + rows * q;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) dst[r] = src[r];
+  }
+}
+
 // out[c + cols r] = in[r + rows c] sc[c]: 64 x 64 tiles through LDS, so both the
 // reads (along r) and the writes (along c) are coalesced.
 __global__ __launch_bounds__(256) void transpose_scale_kernel(const double* __restrict__ in, int64_t rows,
@@ -333,6 +347,14 @@ void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t 
   if (rows <= 0 || ncol <= 0) return;
   const unsigned grid = unsigned(std::min<int64_t>(ncol, 65536));
   hipLaunchKernelGGL(scale_rows_kernel, dim3(grid), dim3(256), 0, s, in, rows, ni, ncol, si, sb, d, cb, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void pair_gather(const double* in, int64_t rows, int a, int c, int j0, int nj, int nk, double* out, hipStream_t s) {
+  const int64_t ncol = int64_t(nj) * nk;
+  if (rows <= 0 || ncol <= 0) return;
+  const unsigned grid = unsigned(std::min<int64_t>(ncol, 65536));
+  hipLaunchKernelGGL(pair_gather_kernel, dim3(grid), dim3(256), 0, s, in, rows, a, c, j0, nk, ncol, out);
   UW12G_HIP(hipGetLastError());
 }
 

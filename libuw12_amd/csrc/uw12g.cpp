@@ -242,6 +242,9 @@ struct uw12g_abs_projectors {
 
 namespace {
 
+// Active j per block of the pair-symmetric L contraction of the same-spin 3e Fock
+constexpr int kPairBlock = 16;
+
 template <class F>
 int guard(F&& f) {
   try {
@@ -539,7 +542,8 @@ void progress(uw12g_context* ctx, const char* fmt, Args... args) {
   size_t free_b = 0, total_b = 0;
   (void)hipMemGetInfo(&free_b, &total_b);
   std::fprintf(stderr, "[uw12g] %8.2fs %6.1f GB free  ", t, free_b / 1e9);
-  std::fprintf(stderr, fmt, args...);
+  if constexpr (sizeof...(Args) == 0) std::fputs(fmt, stderr);
+  else std::fprintf(stderr, fmt, args...);
   std::fprintf(stderr, "\n");
   std::fflush(stderr);
 }
@@ -1224,7 +1228,47 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         nj_max = std::max<int64_t>(nj_max, 1);
         const int64_t n_blocks = (a + nj_max - 1) / nj_max;
         const int nj = int((a + n_blocks - 1) / n_blocks);
-        double* Kt = ctx->buf("blk1", size_t(MNo) * nj);
+        // The second pass has no R, so its L runs over unordered active pairs.  O2_X[(j, c+k), A]
+        // is symmetric in the active j, k (the AO block of Gamma is built from the packed
+        // mu >= nu pairs of the input, so it is symmetric), hence
+        //   sum_{j,k} K~[., c+k | j] X(j, k) = sum_{j <= k} S(j, k) X(j, k),
+        //   S(j, k) = K~[., c+k | j] + K~[., c+j | k] (j < k),  S(j, j) = K~[., c+j | j].
+        // The j run in blocks of jb; block [j0, j0 + nb) keeps the core k, its own active k
+        // (plain K~) and the active k past it (S), nk = c + a - j0 columns per j, so L's k
+        // runs over c a + sum nb (a - j0) pairs instead of o a (about 0.6 o a at jb = 16).
+        struct PairBlock {
+          int j0, nb, nk;
+          int64_t p0;  // first pair of the block in the packed pair order
+        };
+        std::vector<PairBlock> pblk;
+        int64_t n_pairs = 0;
+        const int jb = int(std::min<int64_t>(kPairBlock, nj_max));
+        for (int j0 = 0; j0 < a; j0 += jb) {
+          const int nb = std::min(jb, a - j0), nk = c + a - j0;
+          pblk.push_back({j0, nb, nk, n_pairs});
+          n_pairs += int64_t(nb) * nk;
+        }
+        // groups of consecutive blocks whose sums S fit the block cap together (and, when L
+        // runs in FP32, one FP32 range of its k): one L contraction per group
+        auto pairs_fit = [&](int64_t np) {
+          return int64_t(MrN) * np <= int64_t(cap) && (ctx->prec == 0 || np <= kF32Chunk);
+        };
+        std::vector<std::pair<size_t, size_t>> pgrp;  // [first, end) block indices
+        int64_t np_max = 0;
+        for (size_t i = 0; i < pblk.size();) {
+          size_t e = i;
+          int64_t np = 0;
+          do {
+            np += int64_t(pblk[e].nb) * pblk[e].nk;
+            ++e;
+          } while (e < pblk.size() && pairs_fit(np + int64_t(pblk[e].nb) * pblk[e].nk));
+          pgrp.emplace_back(i, e);
+          np_max = std::max(np_max, np);
+          i = e;
+        }
+        double* Kt = ctx->buf("blk1", std::max(size_t(MNo) * nj, MrN * size_t(np_max)));
+        // X2sP[A + DX p] = X2sT[A + DX (j + a k)] for pair p = (j, k) of the packed order
+        double* X2sP = ctx->buf("x2sp", size_t(n_pairs) * DV);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
         // L contraction below reads its B operand along unit stride
         double* X2sT = ctx->buf("blk3", size_t(ao) * DXmax);
@@ -1243,11 +1287,86 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           const int DX = X->D, DY = Y->D;
           timed(ctx, "3e_ind_fock_x2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DX,
                 [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
+          const bool paired = pass == 1;
+          if (paired)
+            timed(ctx, "3e_ind_fock_x2t", "pair_gather_kernel", 0.0, 16.0 * n_pairs * DX, [&] {
+              for (const PairBlock& pb : pblk)
+                pair_gather(X2sT, DX, a, c, pb.j0, pb.nb, pb.nk, X2sP + int64_t(DX) * pb.p0, st);
+            });
           const int64_t nAb = (DX + nA_max - 1) / nA_max;
           const int nA = int((DX + nAb - 1) / nAb);
           for (int A0 = 0; A0 < DX; A0 += nA) {
             const int An = std::min(nA, DX - A0);
-            for (int j0 = 0; j0 < a; j0 += nj) {
+            for (size_t gi = 0; paired && gi < pgrp.size(); ++gi) {
+              const int64_t pbase = pblk[pgrp[gi].first].p0;
+              for (size_t bi = pgrp[gi].first; bi < pgrp[gi].second; ++bi) {
+                const PairBlock& pb = pblk[bi];
+                double* S = Kt + MrN * size_t(pb.p0 - pbase);  // S[(m, beta) + MrN (kk + nk jj)]
+                const int nbeyond = a - pb.j0 - pb.nb;
+                GemmDesc g;  // S = K~[m, (beta, k) | j], j in the block, k core or active from j0 on
+                g.K = DY;
+                g.batch = pb.nb;
+                g.A = phitY + size_t(Mr) * pb.j0;
+                g.Am = lin(1);
+                g.Ak = lin(int64_t(Mr) * a);
+                g.Ab = Mr;
+                g.Bk = lin(int64_t(N) * o);
+                g.Bn = lin(1);
+                g.Cm = lin(1);
+                g.Cn = lin(Mr);
+                g.Cb = int64_t(MrN) * pb.nk;
+                g.M = Mr;
+                if (c > 0) {
+                  g.N = N * c;
+                  g.B = phiY;
+                  g.C = S;
+                  run_gemm(ctx, g, "3e_ind_fock_kt");
+                }
+                g.N = N * (a - pb.j0);
+                g.B = phiY + size_t(N) * (c + pb.j0);
+                g.C = S + MrN * size_t(c);
+                run_gemm(ctx, g, "3e_ind_fock_kt");
+                if (nbeyond > 0) {
+                  // S[., c + k - j0 | j] += K~[m, (beta, c+j) | k] for the active k past the block:
+                  // rows (m, k), columns (beta, j), one launch
+                  GemmDesc h;
+                  h.M = Mr * nbeyond;
+                  h.N = N * pb.nb;
+                  h.K = DY;
+                  h.beta = 1.0;
+                  h.A = phitY + size_t(Mr) * (pb.j0 + pb.nb);
+                  h.Am = lin(1);
+                  h.Ak = lin(int64_t(Mr) * a);
+                  h.B = phiY + size_t(N) * (c + pb.j0);
+                  h.Bk = lin(int64_t(N) * o);
+                  h.Bn = lin(1);
+                  h.C = S + MrN * size_t(c + pb.nb);
+                  h.Cm = two(uint32_t(Mr), 1, int64_t(MrN));
+                  h.Cn = two(uint32_t(N), Mr, int64_t(MrN) * pb.nk);
+                  run_gemm(ctx, h, "3e_ind_fock_kt_pair");
+                }
+              }
+              const PairBlock& last = pblk[pgrp[gi].second - 1];
+              GemmDesc h;  // L[(m, beta), A] += sum_p S[(m, beta), p] X2sP[A, p] over the group's pairs
+              h.M = int(MrN);
+              h.N = An;
+              h.K = int(last.p0 + int64_t(last.nb) * last.nk - pbase);
+              h.beta = gi == 0 ? 0.0 : 1.0;
+              h.A = Kt;
+              h.Am = lin(1);
+              h.Ak = lin(int64_t(MrN));
+              h.B = X2sP + int64_t(DX) * pbase + A0;
+              h.Bk = lin(DX);
+              h.Bn = lin(1);
+              h.C = L;
+              h.Cm = lin(1);
+              h.Cn = lin(int64_t(MrN));
+              run_gemm(ctx, h, "3e_ind_fock_l");
+              progress(ctx, "same-spin 3e Fock, spin %d pass %d, A %d-%d of %d, pair group %d of %d (j %d-%d)", s,
+                       pass, A0, A0 + An, DX, int(gi) + 1, int(pgrp.size()), pblk[pgrp[gi].first].j0,
+                       last.j0 + last.nb);
+            }
+            for (int j0 = 0; !paired && j0 < a; j0 += nj) {
               const int jn = std::min(nj, a - j0);
               GemmDesc g;  // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B]
               g.M = Mr;
