@@ -32,11 +32,14 @@ int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, do
 
 /* Average time in milliseconds of `reps` launches of the same GEMM on arrays
  * already in device memory (after one untimed launch), on a stream of its own;
- * prec 0 runs the FP64 matrix cores, 1 the FP32 form. */
+ * prec 0 runs the FP64 matrix cores, 1 the FP32 form.  nA, nB, nC and nks are
+ * the element counts of the arrays: a descriptor that would reach past one of
+ * them is refused (-2) before anything is launched. */
 int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
-                    uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, uw12g_map2 Bk, uw12g_map2 Bn,
-                    long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks,
-                    uw12g_map2 Km, int prec, int reps, double* ms);
+                    size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, size_t nB,
+                    uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,
+                    uw12g_map2 Cn, long long Cb, const double* ks, size_t nks, uw12g_map2 Km, int prec, int reps,
+                    double* ms);
 
 /* Host staging size (doubles per buffer) of uw12g_setup_base_integrals_fn on
  * this context; small values force many shell groups (tests). */

@@ -84,10 +84,10 @@ def load():
                                   dp, ctypes.c_size_t, Map2, ctypes.c_int]
     L.uw12g_time_gemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_double, ctypes.c_double,
-                                  vp, Map2, Map2, ctypes.c_longlong,
-                                  vp, Map2, Map2, ctypes.c_longlong,
-                                  vp, Map2, Map2, ctypes.c_longlong,
-                                  vp, Map2, ctypes.c_int, ctypes.c_int, dp]
+                                  vp, ctypes.c_size_t, Map2, Map2, ctypes.c_longlong,
+                                  vp, ctypes.c_size_t, Map2, Map2, ctypes.c_longlong,
+                                  vp, ctypes.c_size_t, Map2, Map2, ctypes.c_longlong,
+                                  vp, ctypes.c_size_t, Map2, ctypes.c_int, ctypes.c_int, dp]
     L.uw12g_test_set_stream_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_set_exact_stages.argtypes = [vp, ctypes.c_char_p]
     L.uw12g_test_set_block_cap.argtypes = [vp, ctypes.c_size_t]

@@ -97,11 +97,26 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
 }
 
 extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta,
-                               const double* A, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B,
-                               uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, uw12g_map2 Cm, uw12g_map2 Cn,
-                               long long Cb, const double* ks, uw12g_map2 Km, int prec, int reps, double* ms) {
+                               const double* A, size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab,
+                               const double* B, size_t nB, uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C,
+                               size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
+                               uw12g_map2 Km, int prec, int reps, double* ms) {
   try {
     if (prec != 0 && prec != 1) throw Error(-1, "prec must be 0 (FP64) or 1 (FP32)");
+    if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw Error(-1, "M, N, K and batch must be positive");
+    // every element the descriptor addresses lies inside the caller's arrays (the
+    // maps are checked non-negative, so the largest offset is the sum of the maxima)
+    auto within = [](const uw12g_map2& m0, const uw12g_map2& m1, int e0, int e1, long long bs, int nb, size_t n,
+                     const char* what) {
+      const Map2 a = to_map(m0), b = to_map(m1);
+      if (a.s0 < 0 || a.s1 < 0 || b.s0 < 0 || b.s1 < 0 || bs < 0) throw Error(-2, std::string(what) + ": negative stride");
+      const int64_t last = int64_t(bs) * (nb - 1) + map_max(a, e0) + map_max(b, e1);
+      if (uint64_t(last) >= uint64_t(n)) throw Error(-2, std::string(what) + ": descriptor reaches past the array");
+    };
+    within(Am, Ak, M, K, Ab, batch, nA, "time_gemm A");
+    within(Bk, Bn, K, N, Bb, batch, nB, "time_gemm B");
+    within(Cm, Cn, M, N, Cb, batch, nC, "time_gemm C");
+    if (ks) within(Km, uw12g_map2{0, 0, 0}, K, 1, 0, 1, nks, "time_gemm ks");
     UW12G_HIP(hipSetDevice(device));
     GemmDesc d;
     d.prec = prec;

@@ -37,9 +37,10 @@ def main():
         rates = []
         for prec in (0, 1):
             ms = ctypes.c_double()
-            rc = L.uw12g_time_gemm(0, M, N, K, batch, 1.0, 0.0, A.data_ptr(), Am, Ak, Ab, B.data_ptr(), Bk, Bn, Bb,
-                                   C.data_ptr(), Cm, Cn, Cb, ks.data_ptr() if ks is not None else None, Km, prec, 3,
-                                   ctypes.byref(ms))
+            rc = L.uw12g_time_gemm(0, M, N, K, batch, 1.0, 0.0, A.data_ptr(), A.numel(), Am, Ak, Ab, B.data_ptr(),
+                                   B.numel(), Bk, Bn, Bb, C.data_ptr(), C.numel(), Cm, Cn, Cb,
+                                   ks.data_ptr() if ks is not None else None, ks.numel() if ks is not None else 0,
+                                   Km, prec, 3, ctypes.byref(ms))
             if rc != 0:
                 raise RuntimeError(L.uw12g_last_error().decode())
             rates.append(f'{2.0 * M * N * K * batch / ms.value / 1e9:6.1f} TF/s ({ms.value:7.2f} ms)')

@@ -61,3 +61,22 @@ def test_context_without_gpu_reports_error():
     rc = L.uw12g_context_create(0, ctypes.byref(h))
     assert rc != 0
     assert L.uw12g_last_error()
+
+
+def test_time_gemm_refuses_descriptors_past_the_arrays():
+    """uw12g_time_gemm checks every operand's extent on the host before it touches a device."""
+    L = _lib.load()
+    lin = lambda s: _lib.Map2(s, 0, 0)
+    a = np.zeros(16)
+    p = a.ctypes.data
+    ms = ctypes.c_double()
+    M, N, K = 8, 6, 5
+
+    def call(nA, nB, nC, Ak=lin(M), batch=1, Ab=0):
+        return L.uw12g_time_gemm(0, M, N, K, batch, 1.0, 0.0, p, nA, lin(1), Ak, Ab, p, nB, lin(N), lin(1), 0,
+                                 p, nC, lin(1), lin(M), 0, None, 0, lin(0), 0, 1, ctypes.byref(ms))
+
+    for args in ((M * K - 1, K * N, M * N), (M * K, K * N - 1, M * N), (M * K, K * N, M * N - 1),
+                 (M * K, K * N, M * N, _lib.Map2(M, 5 * M, 3)), (2 * M * K - 1, K * N, M * N, lin(M), 2, M * K)):
+        assert call(*args) == -2, args
+        assert b'reaches past' in L.uw12g_last_error()
