@@ -1055,39 +1055,35 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         const double fz = c > 0 ? 0.5 : 1.0;  // frozen core: 1/2 (F_act + F_frozen)
         for (int i0 = 0; i0 < a; i0 += ni) {
           const int in = std::min(ni, a - i0);
+          // X_i and Y_i of the block as single products: i joins X's rows (alpha, i) and Y's
+          // columns (i, k), so neither launch pads the short N / o dimension per i
           GemmDesc gx;  // X_i[alpha, (j,k)] = sum_A Phi_W[alpha, c+i, A] sW_A O2_W[(j,k), A]
-          gx.M = N;
+          gx.M = N * in;
           gx.N = int(ao);
           gx.K = DW;
-          gx.batch = in;
           gx.A = q.phiW + size_t(N) * (c + i0);
           gx.Am = lin(1);
           gx.Ak = lin(int64_t(N) * o);
-          gx.Ab = N;
           gx.B = W2s;
           gx.Bk = lin(ao);
           gx.Bn = lin(1);
           gx.C = Xb;
-          gx.Cm = lin(1);
+          gx.Cm = two(uint32_t(N), 1, per_i);
           gx.Cn = lin(N);
-          gx.Cb = per_i;
           run_gemm(ctx, gx, "4e_ind_fock_x");
           GemmDesc gy;  // Y_i[(beta, j), k] = sum_B Phi_V[beta, c+j, B] sV_B O2_V[i + a k, B]
           gy.M = N * a;
-          gy.N = o;
+          gy.N = o * in;
           gy.K = DV;
-          gy.batch = in;
           gy.A = q.phiV + size_t(N) * c;
           gy.Am = lin(1);
           gy.Ak = lin(int64_t(N) * o);
           gy.B = V2sT + int64_t(DV) * i0;
           gy.Bk = lin(1);
-          gy.Bn = lin(int64_t(DV) * a);
-          gy.Bb = DV;
+          gy.Bn = two(uint32_t(in), DV, int64_t(DV) * a);
           gy.C = Yb;
           gy.Cm = lin(1);
-          gy.Cn = lin(int64_t(N) * a);
-          gy.Cb = per_i;
+          gy.Cn = two(uint32_t(in), per_i, int64_t(N) * a);
           run_gemm(ctx, gy, "4e_ind_fock_y");
           GemmDesc gf;  // F -= 2 k1 sum_{(j,k,i)} X[alpha, .] Y[beta, .]
           gf.M = N;
@@ -1401,22 +1397,31 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.Cn = lin(int64_t(MrN));
               run_gemm(ctx, h, "3e_ind_fock_l");
               if (pass == 0 && A0 == 0) {
-                GemmDesc r;  // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j]
+                // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j], with jf
+                // of the block's j as extra columns (alpha, j) of one product: wider than N alone,
+                // while every column offset into the K~ block stays below 2^27 doubles
+                int jf = 1;
+                for (int d = jn; d > 1; --d)
+                  if (jn % d == 0 && int64_t(Mr) * (N - 1) + MNo * (d - 1) < (int64_t(1) << 27)) {
+                    jf = d;
+                    break;
+                  }
+                GemmDesc r;
                 r.M = DW;
-                r.N = N;
+                r.N = N * jf;
                 r.K = Mr * a;
-                r.batch = jn;
+                r.batch = jn / jf;
                 r.A = q.psiW + size_t(Mr) * c;
                 r.Am = lin(int64_t(Mr) * o);
                 r.Ak = lin(1);
                 r.B = Kt + MrN * size_t(c);
                 r.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
-                r.Bn = lin(Mr);
-                r.Bb = MNo;
+                r.Bn = two(uint32_t(N), Mr, MNo);
+                r.Bb = MNo * jf;
                 r.C = R;
                 r.Cm = lin(1);
                 r.Cn = lin(DW);
-                r.Cb = int64_t(DW) * N;
+                r.Cb = int64_t(DW) * N * jf;
                 run_gemm(ctx, r, "3e_ind_fock_r");
                 GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
                 f.M = N;

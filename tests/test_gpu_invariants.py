@@ -170,12 +170,15 @@ def test_recipe_mid_size(gpu_ctx, name, kappa):
     assert fock_error(Fg, Fo) <= 2e-8 * max(1.0, max(np.abs(f).max() for f in Fo))
 
 
-def test_recipe_m12_same_spin(gpu_ctx):
-    print('case m12 kappa=(1.0, 0.5)', flush=True)
+@pytest.mark.parametrize('n_active', [None, [19]])
+def test_recipe_m12_same_spin(gpu_ctx, n_active):
+    """24 occupied: the second same-spin pass pairs its active j in blocks of 16, with the sums S
+    of the pairs past the first block; with 19 active, the 5 core k join every block."""
+    print(f'case m12 kappa=(1.0, 0.5) n_active={n_active}', flush=True)
     s = synth.make_system('m12')
     h = gpu_setup(s, gpu_ctx)
-    eg, Fg = gpu_eval(h, s, 1.0, 0.5)
-    eo, Fo = oracle_eval(s, 1.0, 0.5)
+    eg, Fg = gpu_eval(h, s, 1.0, 0.5, n_active=n_active)
+    eo, Fo = oracle_eval(s, 1.0, 0.5, n_active=n_active)
     for k in TERMS + ('total',):
         assert abs(eg[k] - eo[k]) <= 2e-9 * max(1.0, abs(eo[k])), (k, eg[k], eo[k])
     assert fock_error(Fg, Fo) <= 2e-8 * max(1.0, max(np.abs(f).max() for f in Fo))
