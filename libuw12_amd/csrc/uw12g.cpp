@@ -1299,19 +1299,17 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 const PairBlock& pb = pblk[bi];
                 double* S = Kt + MrN * size_t(pb.p0 - pbase);  // S[(m, beta) + MrN (kk + nk jj)]
                 const int nbeyond = a - pb.j0 - pb.nb;
-                GemmDesc g;  // S = K~[m, (beta, k) | j], j in the block, k core or active from j0 on
+                // S = K~[m, (beta, k) | j], j in the block, k core or active from j0 on; rows (m, j)
+                GemmDesc g;
                 g.K = DY;
-                g.batch = pb.nb;
                 g.A = phitY + size_t(Mr) * pb.j0;
                 g.Am = lin(1);
                 g.Ak = lin(int64_t(Mr) * a);
-                g.Ab = Mr;
                 g.Bk = lin(int64_t(N) * o);
                 g.Bn = lin(1);
-                g.Cm = lin(1);
+                g.Cm = two(uint32_t(Mr), 1, int64_t(MrN) * pb.nk);
                 g.Cn = lin(Mr);
-                g.Cb = int64_t(MrN) * pb.nk;
-                g.M = Mr;
+                g.M = Mr * pb.nb;
                 if (c > 0) {
                   g.N = N * c;
                   g.B = phiY;
@@ -1364,22 +1362,21 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             }
             for (int j0 = 0; !paired && j0 < a; j0 += nj) {
               const int jn = std::min(nj, a - j0);
-              GemmDesc g;  // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B]
-              g.M = Mr;
+              // K~[m, (beta, k) | j] = sum_B (PhiT_Y sY)[m, j, B] Phi_Y[beta, k, B], one product with
+              // rows (m, j): the j of the block extend the ABS rows, so no tile pads Mr per j
+              GemmDesc g;
+              g.M = Mr * jn;
               g.N = N * o;
               g.K = DY;
-              g.batch = jn;
               g.A = phitY + size_t(Mr) * j0;
               g.Am = lin(1);
               g.Ak = lin(int64_t(Mr) * a);
-              g.Ab = Mr;
               g.B = phiY;
               g.Bk = lin(int64_t(N) * o);
               g.Bn = lin(1);
               g.C = Kt;
-              g.Cm = lin(1);
+              g.Cm = two(uint32_t(Mr), 1, MNo);
               g.Cn = lin(Mr);
-              g.Cb = MNo;
               run_gemm(ctx, g, "3e_ind_fock_kt");
               GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
               h.M = int(MrN);
