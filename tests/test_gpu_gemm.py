@@ -44,6 +44,9 @@ CASES = [
     ('thin_k_split', 96, 80, 4000, 1),
     ('batched', 100, 33, 64, 5),
     ('tiny', 3, 5, 2, 2),
+    # large outputs on 128-wide tiles with a partly filled last row / column of tiles
+    ('edge_rows_cols', 900, 800, 70, 1),
+    ('edge_cols_batched', 1000, 960, 33, 2),
 ]
 
 
