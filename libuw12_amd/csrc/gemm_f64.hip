@@ -438,6 +438,34 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
     return;
   }
   double* Cp = d.C + d.Cb * b;
+  if constexpr (PREC == 0) {
+    if (d.beta != 0.0) {
+      // read-modify-write: the old values of a row group are all loaded before
+      // any of its stores, so a wave waits on memory once per row group
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        const int m = m0 + wm0 + 16 * i + lr;
+        if (m >= d.M) continue;
+        const int64_t mo = d.Cm.off(m);
+        int64_t off[NB][4];
+        double old[NB][4];
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int n = ncol(j, r);
+            off[j][r] = n < d.N ? mo + d.Cn.off(n) : -1;
+            old[j][r] = off[j][r] >= 0 ? Cp[off[j][r]] : 0.0;
+          }
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (off[j][r] >= 0) Cp[off[j][r]] = d.alpha * double(acc[i][j][r]) + d.beta * old[j][r];
+      }
+      return;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < MB; ++i) {
     const int m = m0 + wm0 + 16 * i + lr;
