@@ -339,6 +339,24 @@ def main():
         tr, Fr, tp = timed_form(precision, name)
         others[f'{name}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
                                       speedup_vs_fp64=round(t64 / tp, 3))
+    if args.side_forms == 'auto':
+        # the energy-only call (uw12_energy, libuw12.hpp:129-140) in the timed form: no Fock, the
+        # same-spin 3e energy from its own FP64 products instead of the Fock's L
+        def energy_only():
+            return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
+                               indirect_term=indirect, calculate_fock=False, scale_opp_spin=k0,
+                               scale_same_spin=k1, precision=prec)[0]
+        reps = max(1, args.side_reps)
+        energy_only()
+        sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            te = energy_only()
+        sync()
+        tp = (time.perf_counter() - t) / reps
+        print(f'[bench] energy-only ({args.form}): {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
+        others['energy_only'] = {'form': args.form, 'evals_per_s': round(1.0 / tp, 6), 'ms_per_eval': round(1e3 * tp, 3),
+                                 'term_error_eh': {k: te[k] - terms64[k] for k in TERM_KEYS}}
     cpu = None
     if args.cpu_baseline in ('auto', 'full') and not stream:
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
