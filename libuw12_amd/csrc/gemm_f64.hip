@@ -268,6 +268,9 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
         ld(I2{}, k2);
         ld(I3{}, k2);
         __builtin_amdgcn_sched_barrier(0);
+        // the MFMA steps at raised wave priority, so the SIMD's other wave issues
+        // its loads and LDS writes in their shadow (FP64: +1-5% on the c2 shapes)
+        __builtin_amdgcn_s_setprio(1);
         kstep(I0{}, cur);
         kstep(I1{}, cur);
         kstep(I2{}, cur);
@@ -280,6 +283,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
           __builtin_amdgcn_sched_group_barrier(0x008, MB * NB / 2, 0);
         }
         __builtin_amdgcn_sched_barrier(0);
+        __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         continue;
       }
