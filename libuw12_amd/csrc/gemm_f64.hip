@@ -40,12 +40,12 @@ constexpr int NT = 256;
 // with the NN padding (4 doubles / 16 floats) used for the other tiles.
 constexpr int kPadK64 = 1, kPadK32 = 18;
 
-// Tile rows per group of the grouped tile order (Params::group).  On the c2
-// K~ product (425 x 200 tiles of 128 x 128, K = 960) the HBM fetch per launch
-// was 80 GiB with the row-fastest order, for 0.6 GB of operands; grouped, the
-// K~ products of an evaluation run at 124 TF/s instead of 117.  Groups of 4, 8
-// and 16 rows were within 1% there; 4 was 5% faster on the pair sums S (680 x
-// 40 tiles).
+// Tile rows per group of the grouped tile order (Params::group).  On the K~
+// product of n_ao = 320, n_df = 960, 80 occupied (425 x 200 tiles of 128 x 128,
+// K = 960) the HBM fetch per launch was 80 GiB with the row-fastest order, for
+// 0.6 GB of operands; grouped, the K~ products of an evaluation run at 124
+// TF/s instead of 117.  Groups of 4, 8 and 16 rows were within 1% there; 4 was
+// 5% faster on the pair sums S (680 x 40 tiles).
 constexpr int kTileGroup = 4;
 
 struct Params {
@@ -148,13 +148,13 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   const int64_t q = w / tiles;
   const int split = int(q % p.splits);
   const int b = int(q / p.splits);
-  // column-fastest order for tall outputs: the tiles that share an A panel run
-  // together, so the panel is read from HBM once
-  // Grouped order for outputs with many tile rows and columns: the tiles of a
-  // group (p.group tile rows, every tile column) run with the row index fastest,
-  // so the workgroups an XCD runs at once cover a compact block of tiles and
-  // share its A and B panels in L2, instead of every tile row of one column
-  // (each A panel then comes from HBM once per tile column).
+  // Tile order.  Tall outputs with few tile columns: column index fastest, so
+  // the tiles that share an A panel run together and the panel is read from
+  // HBM once.  Many tile rows and columns: groups of p.group tile rows (every
+  // tile column), row index fastest inside a group, so the workgroups an XCD
+  // runs at once cover a compact block of tiles and share its A and B panels
+  // in L2 (row-fastest over all rows brings every A panel from HBM once per
+  // tile column).  Otherwise row index fastest.
   int tm, tn;
   if (p.nfast) {
     tm = tile / p.tiles_n;
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
         ld(I3{}, k2);
         __builtin_amdgcn_sched_barrier(0);
         // the MFMA steps at raised wave priority, so the SIMD's other wave issues
-        // its loads and LDS writes in their shadow (FP64: +1-5% on the c2 shapes)
+        // its loads and LDS writes in their shadow (FP64: +1-5% on the UW12 shapes)
         __builtin_amdgcn_s_setprio(1);
         kstep(I0{}, cur);
         kstep(I1{}, cur);
