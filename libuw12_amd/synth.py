@@ -32,8 +32,9 @@ CONFIGS = {
     'm6': dict(atoms=6, box=10.6, min_sep=2.5, occ=(12, None)),
     'm6u': dict(atoms=6, box=10.6, min_sep=2.5, occ=(12, 11)),
     'm12': dict(atoms=12, box=13.4, min_sep=2.5, occ=(24, None)),
+    'm12u': dict(atoms=12, box=13.4, min_sep=2.5, occ=(24, 23)),
 }
-CONFIG_INDEX = {'c1': 1, 'c2': 2, 'c3': 3, 'c4': 4, 'c5': 5, 'm6': 6, 'm6u': 7, 'm12': 8}
+CONFIG_INDEX = {'c1': 1, 'c2': 2, 'c3': 3, 'c4': 4, 'c5': 5, 'm6': 6, 'm6u': 7, 'm12': 8, 'm12u': 9}
 
 
 def build():

@@ -170,12 +170,13 @@ def test_recipe_mid_size(gpu_ctx, name, kappa):
     assert fock_error(Fg, Fo) <= 2e-8 * max(1.0, max(np.abs(f).max() for f in Fo))
 
 
-@pytest.mark.parametrize('n_active', [None, [19]])
-def test_recipe_m12_same_spin(gpu_ctx, n_active):
+@pytest.mark.parametrize('name,n_active', [('m12', None), ('m12', [19]), ('m12u', None)])
+def test_recipe_m12_same_spin(gpu_ctx, name, n_active):
     """24 occupied: the second same-spin pass pairs its active j in blocks of 16, with the sums S
-    of the pairs past the first block; with 19 active, the 5 core k join every block."""
-    print(f'case m12 kappa=(1.0, 0.5) n_active={n_active}', flush=True)
-    s = synth.make_system('m12')
+    of the pairs past the first block; with 19 active, the 5 core k join every block; open shell
+    (24 / 23) pairs each spin on its own."""
+    print(f'case {name} kappa=(1.0, 0.5) n_active={n_active}', flush=True)
+    s = synth.make_system(name)
     h = gpu_setup(s, gpu_ctx)
     eg, Fg = gpu_eval(h, s, 1.0, 0.5, n_active=n_active)
     eo, Fo = oracle_eval(s, 1.0, 0.5, n_active=n_active)
