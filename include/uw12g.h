@@ -90,6 +90,30 @@ int uw12g_setup_base_integrals_fn(uw12g_context* ctx, const double* X2, uw12g_th
                                   uw12g_three_index_fn three_index_ri, void* user, const size_t* df_sizes,
                                   size_t n_shells, size_t n_ao, size_t n_df, size_t n_ri, uw12g_base_integrals** out);
 
+/* Three-index integrals of the DF functions [df0, df0 + n), produced on the
+ * device, for uw12g_setup_base_integrals_gen: enqueue on `stream` (the
+ * context's hipStream_t) the writes of the column-major block
+ *   out[mu' + m (nu + n_ao j)] = (mu' nu | x | df0 + j),  nu < n_ao, j < n,
+ * with m = n_ao (with_ri == 0: the AO functions mu' < n_ao) or m = n_ao + n_ri
+ * (with_ri != 0: the AO rows, then row n_ao + r = (r nu | x | df0 + j) of RI
+ * function r).  `out` is device memory of the context's GPU; return 0, or
+ * nonzero to fail the calling entry point with -6. */
+typedef int (*uw12g_three_index_gen_fn)(void* user, size_t df0, size_t n, int with_ri, double* out, void* stream);
+
+/* Replaces the function-based BaseIntegrals constructor with its storage flags
+ * (base_integrals.hpp:52-62, store_ao_integrals / store_ri_integrals; blocks
+ * as in integral_functions.hpp:16-55) for integrals produced on the device by
+ * `gen`, with X2 the two-index matrix.  stored != 0: the library builds and
+ * keeps its three-index tensor, as the other setup calls do.  stored == 0:
+ * only the DF metric's eigenvectors are kept, and every evaluation requests
+ * again the blocks it needs (the reference's non-storing, direct mode,
+ * direct_utils.cpp:349-436), so no three-index tensor of the size of X3_ri is
+ * ever resident.  Evaluations with non-stored W / V integrals compute the
+ * opposite-spin terms (indirect_term == 0 or scale_same_spin == 0; -2
+ * otherwise) and take UW12G_FP64 or UW12G_MIXED. */
+int uw12g_setup_base_integrals_gen(uw12g_context* ctx, const double* X2, uw12g_three_index_gen_fn gen, void* user,
+                                   size_t n_ao, size_t n_df, size_t n_ri, int stored, uw12g_base_integrals** out);
+
 void uw12g_base_integrals_destroy(uw12g_base_integrals* ints);
 /* Number of DF metric eigenvectors kept (|lambda| >= 1e-10), or negative status. */
 long uw12g_base_integrals_kept(const uw12g_base_integrals* ints);

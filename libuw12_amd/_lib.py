@@ -20,7 +20,8 @@ class Map2(ctypes.Structure):
 # every symbol declared in include/uw12g.h and include/uw12g_testing.h
 EXPORTS = (
     'uw12g_last_error', 'uw12g_context_create', 'uw12g_context_destroy', 'uw12g_setup_base_integrals',
-    'uw12g_setup_base_integrals_device', 'uw12g_setup_base_integrals_fn', 'uw12g_base_integrals_destroy',
+    'uw12g_setup_base_integrals_device', 'uw12g_setup_base_integrals_fn', 'uw12g_setup_base_integrals_gen',
+    'uw12g_base_integrals_destroy',
     'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
     'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
@@ -57,6 +58,9 @@ def load():
     L.uw12g_setup_base_integrals_fn.argtypes = [vp, dp, THREE_INDEX_FN, THREE_INDEX_FN, vp,
                                                 ctypes.POINTER(ctypes.c_size_t), ctypes.c_size_t, ctypes.c_size_t,
                                                 ctypes.c_size_t, ctypes.c_size_t, ctypes.POINTER(vp)]
+    # the generator is a raw C function pointer (uw12g_three_index_gen_fn), e.g. from libuw12_synth_dev.so
+    L.uw12g_setup_base_integrals_gen.argtypes = [vp, dp, vp, vp, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t,
+                                                 ctypes.c_int, ctypes.POINTER(vp)]
     L.uw12g_base_integrals_destroy.argtypes = [vp]
     L.uw12g_base_integrals_destroy.restype = None
     L.uw12g_base_integrals_kept.argtypes = [vp]

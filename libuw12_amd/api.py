@@ -205,6 +205,24 @@ def setup_base_integrals_fn(X2, three_index, three_index_ri, df_sizes, n_ao, n_r
     return BaseIntegrals(h, ctx, n_ao, n_df, n_ri if three_index_ri is not None else 0)
 
 
+def setup_base_integrals_gen(X2, gen, user, n_ao, n_ri=0, stored=True, context=None, keep=None):
+    """Device-generated integrals (uw12g_setup_base_integrals_gen; the function-based
+    constructor with its storage flags, base_integrals.hpp:52-62).  `gen` is the address of
+    a uw12g_three_index_gen_fn, `user` its user pointer; `keep` is held by the returned
+    object (the generator must outlive it).  stored=False keeps only the DF metric's
+    eigenvectors and regenerates the blocks at every evaluation."""
+    ctx = context or Context.default()
+    X2 = _f64(X2)
+    n_df = X2.shape[0]
+    h = vp()
+    check(_lib.load().uw12g_setup_base_integrals_gen(ctx._h, _ptr(X2), vp(gen), vp(user), n_ao, n_df, n_ri,
+                                                     1 if stored else 0, ctypes.byref(h)))
+    b = BaseIntegrals(h, ctx, n_ao, n_df, n_ri)
+    b._keep = keep
+    b.stored = bool(stored)
+    return b
+
+
 def setup_abs_projectors(S, n_ao, n_ri, context=None):
     """libuw12.hpp:80."""
     ctx = context or Context.default()

@@ -15,7 +15,7 @@ BUILD_DIR = os.path.join(os.path.dirname(HERE), '_build', 'libuw12g')
 ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = os.environ.get('UW12G_ARCH', 'gfx950')
 
-SOURCES = ['gemm_f64.hip', 'kernels.hip', 'uw12g.cpp', 'testing.cpp']
+SOURCES = ['gemm_f64.hip', 'kernels.hip', 'uw12g.cpp', 'uw12g_stream.cpp', 'testing.cpp']
 
 COMMON = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',
@@ -59,6 +59,24 @@ def build(verbose=False):
     return OUT
 
 
+SYNTH_DEV_OUT = os.path.join(HERE, 'libuw12_synth_dev.so')
+
+
+def build_synth_dev():
+    """Device generator of the synthetic s-Gaussian integrals (csrc/synth_dev.hip): the
+    three-index blocks of the benchmark inputs, written straight into the library's
+    device buffers through the uw12g_three_index_gen_fn callback."""
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    tmp = SYNTH_DEV_OUT + '.tmp'
+    cmd = [_hipcc(), *COMMON, '-shared', '-x', 'hip', os.path.join(CSRC, 'synth_dev.hip'), '-o', tmp,
+           f'-Wl,-rpath,{ROCM}/lib']
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f'hipcc failed for synth_dev.hip:\n{r.stderr}')
+    os.replace(tmp, SYNTH_DEV_OUT)
+    return SYNTH_DEV_OUT
+
+
 def build_cpp_driver():
     """C++ caller of include/uw12g.hpp used by the C++ API test (tests/cpp)."""
     root = os.path.dirname(HERE)
@@ -74,4 +92,5 @@ def build_cpp_driver():
 
 if __name__ == '__main__':
     print(build(verbose=True))
+    print(build_synth_dev())
     print(build_cpp_driver())

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <map>
 #include <mutex>
+#include <numeric>
 #include <string>
 #include <type_traits>
 
@@ -618,15 +619,49 @@ namespace {
 // Conditions for the unpredicated interior path of gemm_f64_kernel: affine k
 // maps and every lane byte offset (row or column offset, plus the lane's k
 // offset inside a slab in the k-contiguous forms) below 2^32.
-bool fast_ok(const GemmDesc& d, bool akc, bool bkc) {
+// Largest lane offset (doubles) of the buffer-load path: lane plus uniform byte
+// offset of every load below num_records = 2^31 - 1.
+constexpr int64_t kLaneLim = (int64_t(1) << 28) - 1;
+
+bool fast_maps_ok(const GemmDesc& d) {
   if (!slab_affine(d.Ak) || !slab_affine(d.Bk) || (d.ks && !slab_affine(d.Km))) return false;
-  if (!nonneg(d.Am) || !nonneg(d.Ak) || !nonneg(d.Bk) || !nonneg(d.Bn) || (d.ks && !nonneg(d.Km))) return false;
-  // lane plus uniform byte offset of every buffer load below num_records = 2^31 - 1
-  const int64_t lim = (int64_t(1) << 28) - 1;  // doubles
+  return nonneg(d.Am) && nonneg(d.Ak) && nonneg(d.Bk) && nonneg(d.Bn) && (!d.ks || nonneg(d.Km));
+}
+
+bool fast_ok(const GemmDesc& d, bool akc, bool /*bkc*/) {
+  if (!fast_maps_ok(d)) return false;
   const int64_t a = map_max(d.Am, d.M) + (BK - 1) * d.Ak.s0;
   const int64_t b = map_max(d.Bn, d.N) + (BK - 1) * d.Bk.s0;
   const int64_t k = (d.ks && akc) ? (BK - 1) * d.Km.s0 : 0;
-  return a < lim && b < lim && k < lim;
+  return a < kLaneLim && b < kLaneLim && k < kLaneLim;
+}
+
+// Inner period of a map over [0, extent): n0 for a two-level map that wraps
+// inside the extent, else 1 (the map shifts at any start).
+int64_t period(const Map2& m, int64_t extent) {
+  return (m.n0 == 0xffffffffu || int64_t(m.n0) >= extent) ? 1 : int64_t(m.n0);
+}
+
+// Longest leading range [0, len) of the operand map `a` (len <= extent) whose
+// offsets stay within `budget`, with len a multiple of the periods of `a` and
+// of the output map `c`, so that both restart shifted after it:
+// off(start + i) = off(start) + off(i) for start a multiple of len.  Ranges
+// shorter than the extent are rounded to whole 128-wide tiles where the
+// periods allow.  0: no such range.
+int64_t chunk_len(const Map2& a, const Map2& c, int64_t extent, int64_t budget) {
+  if (budget < 0) return 0;
+  if (map_max(a, extent) <= budget) return extent;
+  const int64_t pa = period(a, extent), pc = period(c, extent);
+  const int64_t al = pa / std::gcd(pa, pc) * pc;
+  int64_t lo = 0, hi = extent / al;  // map_max is nondecreasing in the length (nonnegative strides)
+  while (lo < hi) {
+    const int64_t mid = (lo + hi + 1) / 2;
+    if (map_max(a, mid * al) <= budget) lo = mid;
+    else hi = mid - 1;
+  }
+  int64_t len = lo * al;
+  if (128 % al == 0 && len >= 128) len = len / 128 * 128;
+  return len;
 }
 
 }  // namespace
@@ -754,6 +789,29 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   // coalesce along k only when that index has the unit stride and the other does not
   const bool akc = unit(d.Ak) && !unit(d.Am);
   const bool bkc = unit(d.Bk) && !unit(d.Bn);
+  // Operands whose row / column offsets pass the buffer path's 32-bit lane offsets:
+  // split the output into blocks of rows / columns, each launched from a 64-bit
+  // base pointer (A, B and C shifted to the block's first row / column), so the
+  // lane offsets of every block stay small and the block keeps the fast path.
+  if (d.K >= BK && fast_maps_ok(d) && !fast_ok(d, akc, bkc)) {
+    const int64_t ka = (BK - 1) * d.Ak.s0, kb = (BK - 1) * d.Bk.s0;
+    const bool kfit = !(d.ks && akc) || (BK - 1) * d.Km.s0 < kLaneLim;
+    const int64_t cm = chunk_len(d.Am, d.Cm, d.M, kLaneLim - 1 - ka);
+    const int64_t cn = chunk_len(d.Bn, d.Cn, d.N, kLaneLim - 1 - kb);
+    if (kfit && cm > 0 && cn > 0 && (cm < d.M || cn < d.N)) {
+      for (int64_t m0 = 0; m0 < d.M; m0 += cm)
+        for (int64_t n0 = 0; n0 < d.N; n0 += cn) {
+          GemmDesc c = d;
+          c.M = int(std::min<int64_t>(cm, d.M - m0));
+          c.N = int(std::min<int64_t>(cn, d.N - n0));
+          c.A = d.A + d.Am.off(uint32_t(m0));
+          c.B = d.B + d.Bn.off(uint32_t(n0));
+          c.C = d.C + d.Cm.off(uint32_t(m0)) + d.Cn.off(uint32_t(n0));
+          gemm_f64(c, stream, ws, prof);
+        }
+      return;
+    }
+  }
   // with k along the lanes the unpredicated path needs whole slabs: the k
   // remainder then runs on the general path, accumulated onto them
   const int krem = (akc || bkc) ? d.K % BK : 0;

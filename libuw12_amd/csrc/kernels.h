@@ -26,10 +26,19 @@ struct DotDesc {
 void assemble_gamma(const double* X3p, const double* X3ri, double* G, int N, int Nr, int na, hipStream_t s);
 void gather_scale_cols(const double* Q, int n, const int* keep, const double* scale, double* T, int nk,
                        hipStream_t s);
+// y[c][A] = sum_r G[r + ldg A] x[c][r] (nvec <= 2 vectors); `partial` holds at
+// least min(partial_cap, ...) doubles of per-block sums, added in a fixed order.
 void gemv_t(const double* G, int64_t rows, int64_t ldg, const double* x, int64_t ldx, int nvec, double* y,
-            int64_t ldy, int ncols, hipStream_t s);
+            int64_t ldy, int ncols, double* partial, int64_t partial_cap, hipStream_t s);
+// y[c][r] += sum_A G[r + ldg A] v[c][A] (nvec <= 2); `part` holds the per-column-chunk
+// partial rows (gemv_n_scratch doubles), added in a fixed order.
 void gemv_n(const double* G, int64_t rows, int64_t ldg, const double* v, int64_t ldv, int nvec, double* y,
-            int64_t ldy, int ncols, hipStream_t s);
+            int64_t ldy, int ncols, double* part, int64_t part_cap, hipStream_t s);
+int64_t gemv_n_scratch(int64_t rows, int nvec, int ncols);
+// E += sum_{s,t} 1/2 f[s][t] sum_A XD[A,s] sg[A] XD[A,t];  v[A,s] = sum_t 1/2 ns f[s][t] sg[A] XD[A,t]
+// (v may be null).  One block: D is the DF dimension.
+void xd_combine(const double* XD, int D, int ns, const double f[2][2], const double* sg, double* v, double* E,
+                hipStream_t s);
 // *out = (accumulate ? *out : 0) + scale * dot
 void dot(const DotDesc& d, double scale, double* out, int accumulate, double* partial, hipStream_t s);
 void e4i_energy(const double* W4, const double* V4, int a, int o, double scale, double* out, double* partial,

@@ -60,37 +60,176 @@ __global__ void gather_scale_cols_kernel(const double* Q, int n, const int* keep
   }
 }
 
-// y[c][A] = sum_r G[r + ldg A] x[c][r] for nvec vectors; one block per column A.
-__global__ void gemv_t_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg, const double* __restrict__ x,
-                              int64_t ldx, int nvec, double* __restrict__ y, int64_t ldy, int ncols) {
-  const int A = blockIdx.x;
-  if (A >= ncols) return;
+// y[c][A] = sum_r G[r + ldg A] x[c][r] for nvec <= 2 vectors.  Block (A, part)
+// sums rows [part * rpb, (part + 1) * rpb) of column A into
+// partial[part + parts (A + ncols c)]; gemv_t_finish adds the parts in order.
+// Each lane streams two consecutive rows per load (16 bytes) with four loads
+// in flight.  The rows are read once per pass; x (both vectors) stays in L2.
+constexpr int kGemvUnroll = 4;
+__global__ __launch_bounds__(RB) void gemv_t_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg,
+                                                    const double* __restrict__ x, int64_t ldx, int nvec,
+                                                    double* __restrict__ partial, int parts, int64_t rpb,
+                                                    int ncols) {
+  const int A = blockIdx.x, part = blockIdx.y;
   const double* g = G + ldg * A;
-  double acc[2] = {0.0, 0.0};
-  for (int64_t r = threadIdx.x; r < rows; r += RB) {
-    const double gv = g[r];
-    acc[0] += gv * x[r];
-    if (nvec > 1) acc[1] += gv * x[ldx + r];
+  const int64_t r0 = int64_t(part) * rpb, r1 = min(rows, r0 + rpb);
+  double acc0 = 0.0, acc1 = 0.0;
+  const bool aligned = ((reinterpret_cast<uintptr_t>(g + r0) | reinterpret_cast<uintptr_t>(x + r0) |
+                         (nvec > 1 ? reinterpret_cast<uintptr_t>(x + ldx + r0) : 0)) & 15) == 0;
+  // row pairs (r0 + 2p, r0 + 2p + 1), p < npair, as 16-byte loads, kGemvUnroll per lane in flight
+  const int64_t npair = aligned ? (r1 - r0) / 2 : 0;
+  for (int64_t p = threadIdx.x; p < npair; p += int64_t(RB) * kGemvUnroll) {
+    double2 gv[kGemvUnroll];
+#pragma unroll
+    for (int u = 0; u < kGemvUnroll; ++u) {
+      const int64_t pp = p + int64_t(RB) * u;
+      gv[u] = pp < npair ? *reinterpret_cast<const double2*>(g + r0 + 2 * pp) : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < kGemvUnroll; ++u) {
+      const int64_t pp = p + int64_t(RB) * u;
+      if (pp < npair) {
+        const double2 xv = *reinterpret_cast<const double2*>(x + r0 + 2 * pp);
+        acc0 += gv[u].x * xv.x + gv[u].y * xv.y;
+        if (nvec > 1) {
+          const double2 yv = *reinterpret_cast<const double2*>(x + ldx + r0 + 2 * pp);
+          acc1 += gv[u].x * yv.x + gv[u].y * yv.y;
+        }
+      }
+    }
   }
-  for (int c = 0; c < nvec; ++c) {
-    const double s = block_sum(acc[c]);
-    if (threadIdx.x == 0) y[ldy * c + A] = s;
+  // the rows not covered by pairs (an odd last row, or every row of unaligned operands)
+  for (int64_t q = r0 + 2 * npair + threadIdx.x; q < r1; q += RB) {
+    const double gv = g[q];
+    acc0 += gv * x[q];
+    if (nvec > 1) acc1 += gv * x[ldx + q];
+  }
+  const double s0 = block_sum(acc0);
+  const double s1 = nvec > 1 ? block_sum(acc1) : 0.0;
+  if (threadIdx.x == 0) {
+    partial[part + int64_t(parts) * A] = s0;
+    if (nvec > 1) partial[part + int64_t(parts) * (A + ncols)] = s1;
   }
 }
 
-// y[c][r] += sum_A G[r + ldg A] v[c][A]  (rows split across threads, A loop in order)
-__global__ void gemv_n_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg, const double* __restrict__ v,
-                              int64_t ldv, int nvec, double* __restrict__ y, int64_t ldy, int ncols) {
-  for (int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; r < rows; r += int64_t(gridDim.x) * blockDim.x) {
-    double acc0 = 0.0, acc1 = 0.0;
-    for (int A = 0; A < ncols; ++A) {
-      const double gv = G[r + ldg * A];
-      acc0 += gv * v[A];
-      if (nvec > 1) acc1 += gv * v[ldv + A];
-    }
-    y[r] += acc0;
-    if (nvec > 1) y[ldy + r] += acc1;
+__global__ void gemv_t_finish_kernel(const double* __restrict__ partial, int parts, int nvec, int ncols,
+                                     double* __restrict__ y, int64_t ldy) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < int64_t(ncols) * nvec;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int c = int(e / ncols), A = int(e - int64_t(c) * ncols);
+    const double* p = partial + int64_t(parts) * (A + int64_t(ncols) * c);
+    double s = 0.0;
+    for (int q = 0; q < parts; ++q) s += p[q];
+    y[ldy * c + A] = s;
   }
+}
+
+// Partial y over a chunk of columns: part[(chunk, c)][r] = sum_{A in chunk}
+// G[r + ldg A] v[c][A].  Block (row tile, chunk); each lane owns two
+// consecutive rows (16-byte loads) and runs the chunk's columns four at a time,
+// so eight loads per lane are in flight.  gemv_n_finish adds the chunks in
+// order into y.
+constexpr int kGemvRowsPerBlock = 2 * RB;
+__global__ __launch_bounds__(RB) void gemv_n_kernel(const double* __restrict__ G, int64_t rows, int64_t ldg,
+                                                    const double* __restrict__ v, int64_t ldv, int nvec,
+                                                    double* __restrict__ part, int ncols, int cpb) {
+  const int64_t r = int64_t(blockIdx.x) * kGemvRowsPerBlock + 2 * threadIdx.x;
+  const int chunk = blockIdx.y;
+  const int A0 = chunk * cpb, A1 = min(ncols, A0 + cpb);
+  double a0x = 0.0, a0y = 0.0, a1x = 0.0, a1y = 0.0;
+  if (r + 1 < rows && ((reinterpret_cast<uintptr_t>(G + r) | (uintptr_t(ldg) * 8)) & 15) == 0) {
+    int A = A0;
+    for (; A + kGemvUnroll <= A1; A += kGemvUnroll) {
+      double2 gv[kGemvUnroll];
+#pragma unroll
+      for (int u = 0; u < kGemvUnroll; ++u) gv[u] = *reinterpret_cast<const double2*>(G + r + ldg * (A + u));
+#pragma unroll
+      for (int u = 0; u < kGemvUnroll; ++u) {
+        const double w0 = v[A + u];
+        a0x += gv[u].x * w0;
+        a0y += gv[u].y * w0;
+        if (nvec > 1) {
+          const double w1 = v[ldv + A + u];
+          a1x += gv[u].x * w1;
+          a1y += gv[u].y * w1;
+        }
+      }
+    }
+    for (; A < A1; ++A) {
+      const double2 gv = *reinterpret_cast<const double2*>(G + r + ldg * A);
+      a0x += gv.x * v[A];
+      a0y += gv.y * v[A];
+      if (nvec > 1) {
+        a1x += gv.x * v[ldv + A];
+        a1y += gv.y * v[ldv + A];
+      }
+    }
+  } else {
+    for (int A = A0; A < A1; ++A) {
+      if (r < rows) {
+        const double g0 = G[r + ldg * A];
+        a0x += g0 * v[A];
+        if (nvec > 1) a1x += g0 * v[ldv + A];
+      }
+      if (r + 1 < rows) {
+        const double g1 = G[r + 1 + ldg * A];
+        a0y += g1 * v[A];
+        if (nvec > 1) a1y += g1 * v[ldv + A];
+      }
+    }
+  }
+  double* p = part + rows * (int64_t(chunk) * nvec);
+  if (r < rows) {
+    p[r] = a0x;
+    if (nvec > 1) p[rows + r] = a1x;
+  }
+  if (r + 1 < rows) {
+    p[r + 1] = a0y;
+    if (nvec > 1) p[rows + r + 1] = a1y;
+  }
+}
+
+__global__ void gemv_n_finish_kernel(const double* __restrict__ part, int64_t rows, int nvec, int chunks,
+                                     double* __restrict__ y, int64_t ldy) {
+  for (int64_t e = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; e < rows * nvec;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int c = int(e / rows);
+    const int64_t r = e - int64_t(c) * rows;
+    double s = 0.0;
+    for (int q = 0; q < chunks; ++q) s += part[r + rows * (int64_t(q) * nvec + c)];
+    y[ldy * c + r] += s;
+  }
+}
+
+// Two-electron direct term from X_D (two_electron.cpp:11-81), on one block:
+//   E += sum_{s,t} 1/2 f[s][t] sum_A X_D[A, s] sg[A] X_D[A, t]
+//   v[A, s] = sum_t 1/2 n_spin f[s][t] sg[A] X_D[A, t]   (the Fock's DF vector)
+__global__ __launch_bounds__(RB) void xd_combine_kernel(const double* __restrict__ XD, int D, int ns, double f00,
+                                                        double f01, double f10, double f11,
+                                                        const double* __restrict__ sg, double* __restrict__ v,
+                                                        double* __restrict__ E) {
+  const double f[2][2] = {{f00, f01}, {f10, f11}};
+  double d[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  for (int A = threadIdx.x; A < D; A += RB) {
+    const double x0 = XD[A], x1 = ns > 1 ? XD[D + A] : 0.0, w = sg[A];
+    d[0][0] += x0 * w * x0;
+    if (ns > 1) {
+      d[0][1] += x0 * w * x1;
+      d[1][0] += x1 * w * x0;
+      d[1][1] += x1 * w * x1;
+    }
+    if (v) {
+      v[A] = 0.5 * ns * w * (f[0][0] * x0 + (ns > 1 ? f[0][1] * x1 : 0.0));
+      if (ns > 1) v[D + A] = 0.5 * ns * w * (f[1][0] * x0 + f[1][1] * x1);
+    }
+  }
+  double e = 0.0;
+  for (int s = 0; s < ns; ++s)
+    for (int t = 0; t < ns; ++t) {
+      const double sum = block_sum(d[s][t]);
+      e += 0.5 * f[s][t] * sum;
+    }
+  if (threadIdx.x == 0) *E += e;
 }
 
 // partial[b] = sum over this block's share of  x[i] * y[i] * w(i)
@@ -268,19 +407,55 @@ void gather_scale_cols(const double* Q, int n, const int* keep, const double* sc
 }
 
 void gemv_t(const double* G, int64_t rows, int64_t ldg, const double* x, int64_t ldx, int nvec, double* y,
-            int64_t ldy, int ncols, hipStream_t s) {
+            int64_t ldy, int ncols, double* partial, int64_t partial_cap, hipStream_t s) {
   if (ncols <= 0) return;
   if (nvec > 2) throw Error(-2, "gemv_t supports at most 2 vectors");
-  hipLaunchKernelGGL(gemv_t_kernel, dim3(ncols), dim3(RB), 0, s, G, rows, ldg, x, ldx, nvec, y, ldy, ncols);
+  // enough blocks to fill the chip several times over: columns x row parts
+  int parts = int(std::max<int64_t>(1, std::min<int64_t>((4096 + ncols - 1) / ncols,
+                                                         (rows + 2 * RB * kGemvUnroll - 1) / (2 * RB * kGemvUnroll))));
+  while (parts > 1 && int64_t(parts) * ncols * nvec > partial_cap) --parts;
+  if (int64_t(parts) * ncols * nvec > partial_cap) throw Error(-2, "gemv_t: partial buffer too small");
+  // row parts of whole 16-byte pairs
+  const int64_t rpb = ((rows + parts - 1) / parts + 1) / 2 * 2;
+  parts = int((rows + rpb - 1) / rpb);
+  if (parts < 1) parts = 1;
+  if (parts > 65535) throw Error(-2, "gemv_t: too many row parts");
+  hipLaunchKernelGGL(gemv_t_kernel, dim3(unsigned(ncols), unsigned(parts)), dim3(RB), 0, s, G, rows, ldg, x, ldx,
+                     nvec, partial, parts, rpb, ncols);
+  hipLaunchKernelGGL(gemv_t_finish_kernel, dim3(grid_for(int64_t(ncols) * nvec)), dim3(256), 0, s, partial, parts,
+                     nvec, ncols, y, ldy);
   UW12G_HIP(hipGetLastError());
 }
 
+int64_t gemv_n_scratch(int64_t rows, int nvec, int ncols) {
+  const int64_t tiles = (rows + kGemvRowsPerBlock - 1) / kGemvRowsPerBlock;
+  const int chunks = int(std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, (ncols + 15) / 16)));
+  return rows * nvec * chunks;
+}
+
 void gemv_n(const double* G, int64_t rows, int64_t ldg, const double* v, int64_t ldv, int nvec, double* y,
-            int64_t ldy, int ncols, hipStream_t s) {
-  if (rows <= 0) return;
+            int64_t ldy, int ncols, double* part, int64_t part_cap, hipStream_t s) {
+  if (rows <= 0 || ncols <= 0) return;
   if (nvec > 2) throw Error(-2, "gemv_n supports at most 2 vectors");
-  hipLaunchKernelGGL(gemv_n_kernel, dim3(grid_for(rows, 1 << 20)), dim3(256), 0, s, G, rows, ldg, v, ldv, nvec, y,
-                     ldy, ncols);
+  const int64_t tiles = (rows + kGemvRowsPerBlock - 1) / kGemvRowsPerBlock;
+  if (tiles > 0x7fffffff) throw Error(-2, "gemv_n: too many rows");
+  // column chunks: about 2048 blocks in all, at least 16 columns per chunk
+  int chunks = int(std::max<int64_t>(1, std::min<int64_t>((2048 + tiles - 1) / tiles, (ncols + 15) / 16)));
+  while (chunks > 1 && rows * nvec * chunks > part_cap) --chunks;
+  if (rows * nvec * chunks > part_cap) throw Error(-2, "gemv_n: partial buffer too small");
+  const int cpb = (ncols + chunks - 1) / chunks;
+  chunks = (ncols + cpb - 1) / cpb;
+  hipLaunchKernelGGL(gemv_n_kernel, dim3(unsigned(tiles), unsigned(chunks)), dim3(RB), 0, s, G, rows, ldg, v, ldv,
+                     nvec, part, ncols, cpb);
+  hipLaunchKernelGGL(gemv_n_finish_kernel, dim3(grid_for(rows * nvec, 1 << 16)), dim3(256), 0, s, part, rows, nvec,
+                     chunks, y, ldy);
+  UW12G_HIP(hipGetLastError());
+}
+
+void xd_combine(const double* XD, int D, int ns, const double f[2][2], const double* sg, double* v, double* E,
+                hipStream_t s) {
+  hipLaunchKernelGGL(xd_combine_kernel, dim3(1), dim3(RB), 0, s, XD, D, ns, f[0][0], f[0][1], f[1][0], f[1][1], sg,
+                     v, E);
   UW12G_HIP(hipGetLastError());
 }
 

@@ -34,6 +34,7 @@
 #include "../../include/uw12g.h"
 #include "kernels.h"
 #include "uw12g_internal.h"
+#include "uw12g_impl.h"
 
 namespace uw12g {
 
@@ -105,33 +106,6 @@ Workspace::~Workspace() {
   }
 }
 
-// Device buffer owned by RAII.
-struct DBuf {
-  double* p = nullptr;
-  size_t n = 0;
-  DBuf() = default;
-  DBuf(const DBuf&) = delete;
-  DBuf& operator=(const DBuf&) = delete;
-  ~DBuf() { reset(); }
-  void reset() {
-    if (p) {
-      unregister_alloc(p);
-      (void)hipFree(p);
-    }
-    p = nullptr;
-    n = 0;
-  }
-  double* get(size_t count) {
-    if (count > n || !p) {
-      reset();
-      if (hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)) != hipSuccess)
-        throw Error(-5, "out of device memory (" + std::to_string(count * 8.0 / 1e9) + " GB requested)");
-      n = std::max<size_t>(count, 1);
-      register_alloc(p, n * sizeof(double));
-    }
-    return p;
-  }
-};
 
 void Profiler::begin(ProfileRec& r, hipStream_t s) {
   UW12G_HIP(hipEventCreate(&r.start));
@@ -159,86 +133,76 @@ void Profiler::clear() {
 
 using namespace uw12g;
 
-struct uw12g_context {
-  int device = 0;
-  // doubles allowed for one blocked intermediate of the indirect terms (K~,
-  // Q~, GW/GV blocks): 1/16 of the device memory, at most 16 GiB
-  size_t block_cap = size_t(1) << 28;
-  size_t block_cap_default = size_t(1) << 28;
-  int prec = 0;  // arithmetic of the bulk contractions during an evaluation (GemmDesc::prec)
-  std::vector<std::string> exact_extra;  // stage names also kept FP64 in the mixed form (tests)
-  int verbosity = 0;  // > 0: progress lines on stderr at the major stages of an evaluation
-  std::chrono::steady_clock::time_point t_eval;
-  size_t stream_cap = size_t(1) << 25;  // doubles per host staging buffer of the per-shell setup
-  // Differentiation form: arguments of the last forward call.  Its
-  // intermediates live in the named buffers until the next fused or forward
-  // evaluation on this context (n_eval counts those).
-  struct Tape {
-    uint64_t id = 0, n_eval = 0;
-    const uw12g_base_integrals *W = nullptr, *V = nullptr, *WV = nullptr;
-    const uw12g_abs_projectors* P = nullptr;
-    std::vector<double> C, occ;
-    std::vector<size_t> n_occ, n_active;
-    size_t n_ao = 0, n_orb = 0, n_spin = 0;
-    bool indirect = false;
-    double k0 = 0.0, k1 = 0.0;
-    int prec = 0;
-  } tape;
-  uint64_t n_eval = 0, n_tapes = 0;
-  hipStream_t stream = nullptr;
-  rocblas_handle rb = nullptr;
-  Workspace gemm_ws;
-  std::map<std::string, std::unique_ptr<DBuf>> bufs;
-  Profiler prof;
 
-  // Named device buffers reused across evaluations; growing or releasing one
-  // first drains the stream so no queued kernel still reads the old memory.
-  // "blk1".."blk4" are scratch arenas shared by the consecutive blocked stages
-  // of the indirect terms (4e, 3e energy, 3e Fock i/j, 3e Fock k): each stage
-  // overwrites them in stream order, and keeping them allocated avoids a
-  // multi-GiB allocate/free per evaluation.
-  double* buf(const std::string& name, size_t n) {
-    auto& b = bufs[name];
-    if (!b) b.reset(new DBuf());
-    if (b->p && n > b->n) UW12G_HIP(hipStreamSynchronize(stream));
-    return b->get(n);
+namespace {
+// A handle created on ctx: counted so that the context outlives it.
+void adopt_child(uw12g_context* ctx) { ++ctx->live; }
+// A handle of ctx destroyed: delete the context if it was destroyed meanwhile.
+void release_child(uw12g_context* ctx) {
+  if (--ctx->live == 0 && ctx->doomed) {
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
   }
-  void release(const std::string& name) {
-    auto it = bufs.find(name);
-    if (it == bufs.end()) return;
-    UW12G_HIP(hipStreamSynchronize(stream));
-    bufs.erase(it);
-  }
-  ~uw12g_context() {
-    prof.clear();
-    bufs.clear();
-    if (rb) rocblas_destroy_handle(rb);
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-};
+}
+}  // namespace
 
-struct uw12g_base_integrals {
-  uw12g_context* ctx = nullptr;
-  int N = 0, Nd = 0, Nr = 0, M = 0, D = 0;
-  DBuf gamma;  // M N D
-  DBuf sign;   // D
-  std::vector<double> sign_host;
-  // Gamma rotated onto the kept eigenvectors of one ABS overlap: U^T Gamma
-  // (r N D, built on the first evaluation that pairs these integrals with the
-  // projector whose id is gt_id, replaced when another projector comes)
-  mutable DBuf gt;
-  mutable uint64_t gt_id = 0;
-};
+namespace uw12g {
 
-struct uw12g_abs_projectors {
-  uw12g_context* ctx = nullptr;
-  int N = 0, Nr = 0, M = 0;
-  int r = 0;       // kept eigenvectors of S
-  uint64_t id = 0;  // distinct for every projector created in the process
-  DBuf Sp;    // M x M, S+ = U diag(dinv) U^T
-  DBuf U;     // M x r kept eigenvectors
-  DBuf dinv;  // r: 1 / eigenvalue
-};
+// GEMM launch (timed per launch when profiling is on) in the context's
+// current contraction precision.
+// `exact` contractions stay FP64 in the mixed form: those whose results feed
+// the energies directly (orbital transforms, t, X, the energy-only products)
+// and those that accumulate into the Fock matrix (long sums of terms that
+// cancel).  The mixed form runs the rest -- the large intermediate tensors of
+// the Fock terms (K~, L, R, Xi, G, X_i / Y_i) -- on FP32 matrix cores.
+void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  ctx->prof.stage = stage;
+  GemmDesc e = d;
+  if (ctx->prec == UW12G_MIXED && !exact)
+    for (const std::string& x : ctx->exact_extra)
+      if (x == stage) exact = true;
+  e.prec = ctx->prec == UW12G_MIXED ? (exact ? 0 : 1) : ctx->prec;
+  gemm_f64(e, ctx->stream, &ctx->gemm_ws, &ctx->prof);
+}
+
+// U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
+// eigenvectors U of the ABS overlap.  With S+ = U diag(1/s) U^T every
+// contraction over the ABS index runs over those r rows, S+ becoming the
+// diagonal 1/s (ri_utils.hpp:45-74 drops the other M - r directions).  Built
+// in FP64 once per (integrals, projector) pair and kept on the integrals.
+void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P) {
+  if (X->gt_id == P->id) return;
+  const int M = P->M, N = X->N, Mr = P->r, D = X->D;
+  X->gt_id = 0;
+  if (X->gt.p && size_t(Mr) * N * D > X->gt.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
+  X->gt.get(size_t(std::max(Mr, 1)) * N * std::max(D, 1));
+  if (Mr > 0 && D > 0) {
+    const int prec = ctx->prec;
+    ctx->prec = 0;
+    GemmDesc g;  // Gt[m, alpha, A] = sum_mu' U[mu', m] Gamma[mu', alpha, A]
+    g.M = Mr;
+    g.N = N;
+    g.K = M;
+    g.batch = D;
+    g.A = P->U.p;
+    g.Am = lin(M);
+    g.Ak = lin(1);
+    g.B = X->gamma.p;
+    g.Bk = lin(1);
+    g.Bn = lin(M);
+    g.Bb = int64_t(M) * N;
+    g.C = X->gt.p;
+    g.Cm = lin(1);
+    g.Cn = lin(Mr);
+    g.Cb = int64_t(Mr) * N;
+    run_gemm(ctx, g, "abs_rotate");
+    ctx->prec = prec;
+  }
+  X->gt_id = P->id;
+}
+
+}  // namespace uw12g
 
 namespace {
 
@@ -262,42 +226,9 @@ int guard(F&& f) {
   }
 }
 
-void require(bool ok, const std::string& msg, int code = -1) {
-  if (!ok) throw Error(code, msg);
-}
 
-void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)); }
 
-// GEMM launch (timed per launch when profiling is on) in the context's
-// current contraction precision.
-// `exact` contractions stay FP64 in the mixed form: those whose results feed
-// the energies directly (orbital transforms, t, X, the energy-only products)
-// and those that accumulate into the Fock matrix (long sums of terms that
-// cancel).  The mixed form runs the rest -- the large intermediate tensors of
-// the Fock terms (K~, L, R, Xi, G, X_i / Y_i) -- on FP32 matrix cores.
-void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact = false) {
-  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  ctx->prof.stage = stage;
-  GemmDesc e = d;
-  if (ctx->prec == UW12G_MIXED && !exact)
-    for (const std::string& x : ctx->exact_extra)
-      if (x == stage) exact = true;
-  e.prec = ctx->prec == UW12G_MIXED ? (exact ? 0 : 1) : ctx->prec;
-  gemm_f64(e, ctx->stream, &ctx->gemm_ws, &ctx->prof);
-}
 
-template <class F>
-void timed(uw12g_context* ctx, const char* stage, const char* kernel, double flops, double bytes, F&& f) {
-  if (!ctx->prof.on) {
-    f();
-    return;
-  }
-  ProfileRec rec;
-  ctx->prof.stage = stage;
-  ctx->prof.begin(rec, ctx->stream);
-  f();
-  ctx->prof.end(rec, ctx->stream, kernel, flops, bytes);
-}
 
 // FP64 symmetric eigendecomposition on the device (rocSOLVER syevd, lower
 // triangle, ascending eigenvalues -- the LAPACK routine behind Armadillo's
@@ -341,6 +272,8 @@ struct ThreeIndexSource {
   uw12g_three_index_fn fn = nullptr, fn_ri = nullptr;
   void* user = nullptr;
   std::vector<size_t> shells;  // DF shell sizes (callbacks)
+  uw12g_three_index_gen_fn gen = nullptr;  // device generator
+  bool stored = true;                      // false: keep T, not Gamma (device generator only)
 };
 
 struct PinnedBuf {  // page-locked host staging (hipHostMalloc)
@@ -358,12 +291,12 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
   require(ctx != nullptr, "null context");
   require(out != nullptr, "null output handle");
   require(X2 != nullptr, "null integral array");
-  const bool cb = src.fn != nullptr;
-  require(cb || src.X3 != nullptr, "null integral array");
+  const bool cb = src.fn != nullptr, gen = src.gen != nullptr;
+  require(cb || gen || src.X3 != nullptr, "null integral array");
   require(n_ao > 0, "Number of ao orbitals is zero");
   require(n_df > 0, "Number of density-fitting basis functions is zero");
   if (cb) require((src.fn_ri == nullptr) == (n_ri == 0), "three_index_ri and n_ri must both be given or both be absent");
-  else require((src.X3ri == nullptr) == (n_ri == 0), "X3_ri and n_ri must both be given or both be absent");
+  else if (!gen) require((src.X3ri == nullptr) == (n_ri == 0), "X3_ri and n_ri must both be given or both be absent");
   require(n_ao < 46341 && n_df < (1u << 30) && n_ri < (1u << 30), "size out of range", -2);
   if (cb) {
     size_t tot = 0;
@@ -414,10 +347,23 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
     (void)hipFree(keep_d);
   }
   Q.reset();
+  if (!src.stored) {  // non-stored: T and the generator are all an evaluation needs
+    I->stored = false;
+    I->gen = src.gen;
+    I->gen_user = src.user;
+    I->T.get(size_t(Nd) * std::max(D, 1));
+    if (D > 0) UW12G_HIP(hipMemcpyAsync(I->T.p, T.p, sizeof(double) * Nd * D, hipMemcpyDeviceToDevice, s));
+    I->gamma.get(1);
+    UW12G_HIP(hipStreamSynchronize(s));
+    adopt_child(ctx);
+    *out = I.release();
+    return;
+  }
   // ---- Gamma = G0 T, streamed over DF column blocks of the input
   const size_t MN = size_t(M) * N;
   I->gamma.get(MN * std::max(D, 1));
   if (D == 0) {
+    adopt_child(ctx);
     *out = I.release();
     return;
   }
@@ -450,7 +396,7 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
   for (auto& b : blocks) wmax = std::max(wmax, b.second);
   DBuf G0, st3, stri;
   G0.get(MN * wmax);
-  if (!src.device) {
+  if (!src.device && !gen) {
     st3.get(P * wmax);
     if (Nr > 0) stri.get(PR * wmax);
   }
@@ -463,7 +409,11 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
     const int A0 = blocks[bi].first, na = blocks[bi].second;
     const double* x3 = nullptr;
     const double* xri = nullptr;
-    if (src.device) {
+    if (gen) {
+      const int rc = src.gen(src.user, size_t(A0), size_t(na), Nr > 0 ? 1 : 0, G0.p, s);
+      require(rc == 0, "three-index generator failed for DF functions " + std::to_string(A0) + ".." +
+                           std::to_string(A0 + na), -6);
+    } else if (src.device) {
       x3 = src.X3 + P * A0;
       xri = Nr > 0 ? src.X3ri + PR * A0 : nullptr;
     } else if (!cb) {
@@ -489,7 +439,7 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
       x3 = st3.p;
       xri = stri.p;
     }
-    assemble_gamma(x3, xri, G0.p, N, Nr, na, s);
+    if (!gen) assemble_gamma(x3, xri, G0.p, N, Nr, na, s);
     GemmDesc g;
     g.M = int(MN);
     g.N = D;
@@ -508,89 +458,11 @@ void setup_base(uw12g_context* ctx, const ThreeIndexSource& src, const double* X
     gemm_f64(g, s, &ctx->gemm_ws);
   }
   UW12G_HIP(hipStreamSynchronize(s));
+  adopt_child(ctx);
   *out = I.release();
 }
 
 // ------------------------------------------------------------------ eval
-struct SpinData {
-  int o = 0, a = 0, c = 0;
-  double* Co = nullptr;  // N x o (device)
-  double* wk = nullptr;  // o   (1 + [k active]) when a > 0 else 1
-  double* phiW = nullptr;
-  double* phiV = nullptr;  // N o D: AO rows of Gamma Co
-  double* psiW = nullptr;
-  double* psiV = nullptr;  // Mr o D: (U^T Gamma) Co, the ABS rows rotated onto S's kept eigenvectors
-  double* o2W = nullptr;
-  double* o2V = nullptr;  // a o D
-  double* phitV = nullptr;  // Mr a D_V: diag(1/s) Psi_V[:, act]
-  double* t = nullptr;      // D_W x D_V
-  double* x = nullptr;      // D_W x D_V
-};
-
-double spin_factor(int ns, int s, int t, double k0, double k1) {  // four_electron_utils.cpp:9-30
-  if (ns == 1) return 2.0 * (k0 + k1);
-  return s == t ? k1 : k0;
-}
-
-// A progress line on stderr once the stream has finished the work queued so
-// far (verbosity > 0 only: the wait serialises the host with the GPU).
-template <class... Args>
-void progress(uw12g_context* ctx, const char* fmt, Args... args) {
-  if (ctx->verbosity <= 0) return;
-  UW12G_HIP(hipStreamSynchronize(ctx->stream));
-  const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - ctx->t_eval).count();
-  size_t free_b = 0, total_b = 0;
-  (void)hipMemGetInfo(&free_b, &total_b);
-  std::fprintf(stderr, "[uw12g] %8.2fs %6.1f GB free  ", t, free_b / 1e9);
-  if constexpr (sizeof...(Args) == 0) std::fputs(fmt, stderr);
-  else std::fprintf(stderr, fmt, args...);
-  std::fprintf(stderr, "\n");
-  std::fflush(stderr);
-}
-
-// U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
-// eigenvectors U of the ABS overlap.  With S+ = U diag(1/s) U^T every
-// contraction over the ABS index runs over those r rows, S+ becoming the
-// diagonal 1/s (ri_utils.hpp:45-74 drops the other M - r directions).  Built
-// in FP64 once per (integrals, projector) pair and kept on the integrals.
-void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P) {
-  if (X->gt_id == P->id) return;
-  const int M = P->M, N = X->N, Mr = P->r, D = X->D;
-  X->gt_id = 0;
-  if (X->gt.p && size_t(Mr) * N * D > X->gt.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
-  X->gt.get(size_t(std::max(Mr, 1)) * N * std::max(D, 1));
-  if (Mr > 0 && D > 0) {
-    const int prec = ctx->prec;
-    ctx->prec = 0;
-    GemmDesc g;  // Gt[m, alpha, A] = sum_mu' U[mu', m] Gamma[mu', alpha, A]
-    g.M = Mr;
-    g.N = N;
-    g.K = M;
-    g.batch = D;
-    g.A = P->U.p;
-    g.Am = lin(M);
-    g.Ak = lin(1);
-    g.B = X->gamma.p;
-    g.Bk = lin(1);
-    g.Bn = lin(M);
-    g.Bb = int64_t(M) * N;
-    g.C = X->gt.p;
-    g.Cm = lin(1);
-    g.Cn = lin(Mr);
-    g.Cb = int64_t(Mr) * N;
-    run_gemm(ctx, g, "abs_rotate");
-    ctx->prec = prec;
-  }
-  X->gt_id = P->id;
-}
-
-// What one evaluate() call does: the fused energy (+ optional Fock) pass, the
-// forward half of the differentiation form (energy only; the W/V transforms
-// Phi, O2, t, S+ Phi_V and X stay in the context's named buffers), or its
-// adjoint half (Fock only, reading those buffers instead of recomputing them
-// and skipping the energy-only contractions of the indirect terms).
-enum EvalMode { kFused = 0, kForward = 1, kAdjoint = 2 };
-
 void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
               const uw12g_base_integrals* WV, const uw12g_abs_projectors* P, const double* C, size_t n_ao,
               size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ, const size_t* n_active,
@@ -612,8 +484,17 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
   const double e = ns == 1 ? 2.0 : 1.0;
   ctx->prof.clear();
   ctx->t_eval = std::chrono::steady_clock::now();
-  ensure_rotated(ctx, W, P);
-  ensure_rotated(ctx, V, P);
+  const bool streamed = !W->stored || !V->stored;
+  if (streamed) {
+    require(mode == kFused, "the differentiation form needs stored W and V integrals", -2);
+    require(!do_ind, "the same-spin terms need stored W and V integrals (indirect_term = 0 or scale_same_spin = 0)",
+            -2);
+    require(ctx->prec != UW12G_FP32, "the FP32 form needs stored W and V integrals", -2);
+  } else {
+    require(WV->stored, "non-stored WV integrals need non-stored W and V integrals", -2);
+    ensure_rotated(ctx, W, P);
+    ensure_rotated(ctx, V, P);
+  }
 
   // ---- host: occupation-weighted orbitals and active block (utils.hpp:235-310)
   std::vector<SpinData> sd(ns);
@@ -655,6 +536,27 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     UW12G_HIP(hipMemcpyAsync(sd[s].wk, wk_h[s].data(), sizeof(double) * std::max(o, 1), hipMemcpyHostToDevice, st));
   }
   const size_t NN = size_t(N) * N;
+  auto results = [&] {
+    double Eh[8];
+    UW12G_HIP(hipMemcpyAsync(Eh, Eacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
+    if (calc_fock) {
+      double* Fs = ctx->buf("Fsym", NN * ns);
+      for (int s = 0; s < ns; ++s) symmetrise_add(Fraw + NN * s, Fs + NN * s, N, 0, st);
+      UW12G_HIP(hipMemcpyAsync(fock, Fs, sizeof(double) * NN * ns, hipMemcpyDeviceToHost, st));
+    }
+    UW12G_HIP(hipStreamSynchronize(st));
+    for (int i = 0; i < 6; ++i) terms[i] = Eh[i];
+    if (energy) {
+      double tot = 0.0;
+      for (int i = 0; i < 6; ++i) tot += terms[i];
+      *energy = tot;
+    }
+  };
+  if (streamed) {
+    evaluate_streamed(ctx, W, V, WV, P, sd, k0, k1, Fraw, Eacc, partial);
+    results();
+    return;
+  }
 
   // ================= two-electron term (two_electron.cpp:11-152) =================
   {
@@ -679,30 +581,21 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     }
     // X_D[s]_A = sum_j (jj|A~)  (integrals.cpp:347-378); HBM-bound pass over Gamma_WV
     double* XD = ctx->buf("XD", size_t(std::max(DWV, 1)) * ns);
+    // partial sums of the two passes: gemv_t keeps at most ~4096 + D_WV per vector
+    const int64_t gpart_n = std::max<int64_t>(2 * (4096 + int64_t(DWV)), calc_fock ? gemv_n_scratch(NN, ns, DWV) : 0);
+    double* gpart = ctx->buf("gemv_part", size_t(gpart_n));
     timed(ctx, "2e_XD", "gemv_t_kernel", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
-      gemv_t(WV->gamma.p, int64_t(NN), int64_t(NN), DCa, int64_t(NN), ns, XD, DWV, DWV, st);
+      gemv_t(WV->gamma.p, int64_t(NN), int64_t(NN), DCa, int64_t(NN), ns, XD, DWV, DWV, gpart, gpart_n, st);
     });
-    std::vector<double> XDh(size_t(std::max(DWV, 1)) * ns);
-    UW12G_HIP(hipMemcpyAsync(XDh.data(), XD, sizeof(double) * DWV * ns, hipMemcpyDeviceToHost, st));
-    UW12G_HIP(hipStreamSynchronize(st));
-    double E2d = 0.0;
-    std::vector<double> vh(size_t(std::max(DWV, 1)) * ns, 0.0);
+    // E2_direct and the Fock's DF vector on the device (no host round trip)
+    double f2[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
     for (int s = 0; s < ns; ++s)
-      for (int t = 0; t < ns; ++t) {
-        const double f = spin_factor(ns, s, t, k0, k1);
-        if (f == 0) continue;
-        double d = 0.0;
-        for (int A = 0; A < DWV; ++A) d += XDh[size_t(DWV) * s + A] * WV->sign_host[A] * XDh[size_t(DWV) * t + A];
-        E2d += 0.5 * f * d;
-        for (int A = 0; A < DWV; ++A)
-          vh[size_t(DWV) * s + A] += 0.5 * ns * f * WV->sign_host[A] * XDh[size_t(DWV) * t + A];
-      }
-    terms[UW12G_E2_DIRECT] = E2d;
+      for (int t = 0; t < ns; ++t) f2[s][t] = spin_factor(ns, s, t, k0, k1);
+    double* vd = calc_fock ? ctx->buf("XDv", size_t(std::max(DWV, 1)) * ns) : nullptr;
+    if (DWV > 0) xd_combine(XD, DWV, ns, f2, WV->sign.p, vd, Eacc + UW12G_E2_DIRECT, st);
     if (calc_fock && DWV > 0) {
-      double* vd = ctx->buf("XDv", size_t(DWV) * ns);
-      UW12G_HIP(hipMemcpyAsync(vd, vh.data(), sizeof(double) * DWV * ns, hipMemcpyHostToDevice, st));
       timed(ctx, "2e_fock", "gemv_n_kernel", 2.0 * NN * DWV * ns, 8.0 * NN * DWV, [&] {
-        gemv_n(WV->gamma.p, int64_t(NN), int64_t(NN), vd, DWV, ns, Fraw, int64_t(NN), DWV, st);
+        gemv_n(WV->gamma.p, int64_t(NN), int64_t(NN), vd, DWV, ns, Fraw, int64_t(NN), DWV, gpart, gpart_n, st);
       });
     }
     if (do_ind) {
@@ -797,7 +690,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
   };
   // A fused evaluation of two spins keeps one set of the large ABS-row buffers (Psi, PhiT_V)
   // and re-forms Psi per spin for the same-spin terms; forward / adjoint keep both (the tape).
+  // The shared set is sized for the larger occupied / active count of the two spins up front:
+  // growing it for the second spin would free the memory the first spin's pointers still name.
   const bool share = mode == kFused && ns == 2;
+  int o_max = 0, a_max = 0;
+  for (int s = 0; s < ns; ++s) {
+    o_max = std::max(o_max, sd[s].o);
+    a_max = std::max(a_max, sd[s].a);
+  }
 
   // ================= W/V transforms: Phi, O2, t, PhiT_V, X =================
   for (int s = 0; s < ns; ++s) {
@@ -806,13 +706,14 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     const std::string sfx = std::to_string(s), psfx = share ? std::string("s") : sfx;
     q.phiW = ctx->buf("phiW" + sfx, size_t(N) * o * DW);
     q.phiV = ctx->buf("phiV" + sfx, size_t(N) * o * DV);
-    q.psiW = ctx->buf("psiW" + psfx, size_t(Mr) * o * DW);
-    q.psiV = ctx->buf("psiV" + psfx, size_t(Mr) * o * DV);
+    const int o_buf = share ? o_max : o, a_buf = share ? a_max : a;
+    q.psiW = ctx->buf("psiW" + psfx, size_t(Mr) * o_buf * DW);
+    q.psiV = ctx->buf("psiV" + psfx, size_t(Mr) * o_buf * DV);
     q.o2W = ctx->buf("o2W" + sfx, size_t(a) * o * DW);
     q.o2V = ctx->buf("o2V" + sfx, size_t(a) * o * DV);
     q.t = ctx->buf("t" + sfx, size_t(DW) * DV);
     q.x = ctx->buf("x" + sfx, size_t(DW) * DV);
-    q.phitV = ctx->buf("phitV" + psfx, size_t(Mr) * a * DV);
+    q.phitV = ctx->buf("phitV" + psfx, size_t(Mr) * a_buf * DV);
     if (mode == kAdjoint) continue;  // kept from the forward call
     const double* Ca = q.Co + size_t(N) * c;
     for (int op = 0; op < 2; ++op) {
@@ -1494,21 +1395,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     }
   }
 
-  // ---- results
-  double Eh[8];
-  UW12G_HIP(hipMemcpyAsync(Eh, Eacc, sizeof(double) * 8, hipMemcpyDeviceToHost, st));
-  if (calc_fock) {
-    double* Fs = ctx->buf("Fsym", NN * ns);
-    for (int s = 0; s < ns; ++s) symmetrise_add(Fraw + NN * s, Fs + NN * s, N, 0, st);
-    UW12G_HIP(hipMemcpyAsync(fock, Fs, sizeof(double) * NN * ns, hipMemcpyDeviceToHost, st));
-  }
-  UW12G_HIP(hipStreamSynchronize(st));
-  for (int i = 1; i < 6; ++i) terms[i] = Eh[i];
-  if (energy) {
-    double tot = 0.0;
-    for (int i = 0; i < 6; ++i) tot += terms[i];
-    *energy = tot;
-  }
+  results();
 }
 
 }  // namespace
@@ -1540,6 +1427,10 @@ int uw12g_context_create(int device, uw12g_context** out) {
 
 void uw12g_context_destroy(uw12g_context* ctx) {
   if (!ctx) return;
+  if (ctx->live > 0) {  // deleted with its last integral / projector handle
+    ctx->doomed = true;
+    return;
+  }
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   delete ctx;
@@ -1584,11 +1475,26 @@ int uw12g_setup_base_integrals_fn(uw12g_context* ctx, const double* X2, uw12g_th
   });
 }
 
+int uw12g_setup_base_integrals_gen(uw12g_context* ctx, const double* X2, uw12g_three_index_gen_fn gen, void* user,
+                                   size_t n_ao, size_t n_df, size_t n_ri, int stored, uw12g_base_integrals** out) {
+  return guard([&] {
+    require(gen != nullptr, "null three-index generator");
+    ThreeIndexSource src;
+    src.gen = gen;
+    src.user = user;
+    src.stored = stored != 0;
+    setup_base(ctx, src, X2, n_ao, n_df, n_ri, out);
+  });
+}
+
 void uw12g_base_integrals_destroy(uw12g_base_integrals* ints) {
   if (!ints) return;
-  (void)hipSetDevice(ints->ctx->device);
-  (void)hipStreamSynchronize(ints->ctx->stream);
+  uw12g_context* ctx = ints->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->tape.W == ints || ctx->tape.V == ints || ctx->tape.WV == ints) ctx->tape.id = 0;
   delete ints;
+  release_child(ctx);
 }
 
 long uw12g_base_integrals_kept(const uw12g_base_integrals* ints) { return ints ? long(ints->D) : -1; }
@@ -1662,15 +1568,19 @@ int uw12g_setup_abs_projectors(uw12g_context* ctx, const double* S, size_t n_ao,
       (void)hipFree(keep_d);
     }
     UW12G_HIP(hipStreamSynchronize(s));
+    adopt_child(ctx);
     *out = P.release();
   });
 }
 
 void uw12g_abs_projectors_destroy(uw12g_abs_projectors* abs) {
   if (!abs) return;
-  (void)hipSetDevice(abs->ctx->device);
-  (void)hipStreamSynchronize(abs->ctx->stream);
+  uw12g_context* ctx = abs->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->tape.P == abs) ctx->tape.id = 0;
   delete abs;
+  release_child(ctx);
 }
 
 int uw12g_abs_projectors_get(const uw12g_abs_projectors* abs, double* out) {

@@ -13,7 +13,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, 'libuw12_synth.so')
+DEV_LIB = os.path.join(HERE, 'libuw12_synth_dev.so')
 _lib = None
+_dev_lib = None
 
 # per-atom basis recipe: (count, first exponent, ratio)
 AO = (8, 0.1, 2.5)
@@ -233,3 +235,88 @@ def shell_integrals(system, kind):
                               np.ascontiguousarray(cdf[a:b]))
 
     return two_index(kind, edf, cdf), sizes, fn, (fn_ri if kind != 'WV' else None)
+
+
+def dev_lib():
+    """libuw12_synth_dev.so: the device generator (csrc/synth_dev.hip), built by
+    libuw12_amd.build.build_synth_dev()."""
+    global _dev_lib
+    if _dev_lib is None:
+        if not os.path.exists(DEV_LIB):
+            raise RuntimeError(f'{DEV_LIB} is not built; run `python -m libuw12_amd.build`')
+        L = ctypes.CDLL(DEV_LIB)
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.uw12s_create.argtypes = [ctypes.c_int, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp, ctypes.c_int, dp, dp,
+                                   ctypes.c_int, ctypes.c_int, dp, dp, ctypes.POINTER(ctypes.c_void_p)]
+        L.uw12s_destroy.argtypes = [ctypes.c_void_p]
+        L.uw12s_destroy.restype = None
+        L.uw12s_generate.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p,
+                                     ctypes.c_void_p]
+        L.uw12s_generate_host.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, dp]
+        _dev_lib = L
+    return _dev_lib
+
+
+class DeviceGenerator:
+    """Three-index integrals of one operator of a make_system() system, produced on
+    the device block by block (the uw12g_three_index_gen_fn callback of
+    uw12g_setup_base_integrals_gen).  `fn` is the callback's address and `user`
+    its user pointer; the object must outlive every integrals handle built on it."""
+
+    def __init__(self, system, kind, device=0):
+        L = dev_lib()
+        eao, cao = system['basis']['ao']
+        edf, cdf = system['basis']['df']
+        eri, cri = system['basis']['ri']
+        if kind == 'WV':
+            eri, cri = np.zeros(0), np.zeros((0, 3))
+        c = np.array(GEMINAL[0], dtype=np.float64)
+        g = np.array(GEMINAL[1], dtype=np.float64)
+        arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (eao, cao, edf, cdf, eri, cri)]
+        self._arrs = arrs
+        h = ctypes.c_void_p()
+        rc = L.uw12s_create(int(device), KINDS[kind], _p(arrs[0]), _p(arrs[1]), len(eao), _p(arrs[2]), _p(arrs[3]),
+                            len(edf), _p(arrs[4]) if len(eri) else None, _p(arrs[5]) if len(eri) else None,
+                            len(eri), len(c), _p(c), _p(g), ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f'uw12s_create failed ({rc})')
+        self._h = h
+        self.user = h.value
+        self.fn = ctypes.cast(L.uw12s_generate, ctypes.c_void_p).value
+        self.n_ao, self.n_df, self.n_ri = len(eao), len(edf), len(eri)
+
+    def block(self, df0, n, with_ri):
+        """Host copy [mu', nu, j] of the block the callback writes for DF functions [df0, df0 + n)."""
+        rows = self.n_ao + (self.n_ri if with_ri else 0)
+        out = np.zeros(rows * self.n_ao * n)
+        rc = dev_lib().uw12s_generate_host(self._h, df0, n, int(bool(with_ri)), _p(out))
+        if rc != 0:
+            raise RuntimeError(f'uw12s_generate_host failed ({rc})')
+        return out.reshape(n, self.n_ao, rows).transpose(2, 1, 0)
+
+    def close(self):
+        if getattr(self, '_h', None):
+            dev_lib().uw12s_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_integrals(system, ctx, stored=True, device=0):
+    """W, V and WV integrals of a make_system() system (built with or without host
+    integrals) through the device generator: (W, V, WV) handles, each holding its
+    generator.  stored=False gives the non-stored (regenerating) form for W and V;
+    WV follows the same choice."""
+    import libuw12_amd as u
+    edf, cdf = system['basis']['df']
+    out = []
+    for kind in ('W', 'V', 'WV'):
+        gen = DeviceGenerator(system, kind, device)
+        X2 = two_index(kind, edf, cdf)
+        out.append(u.setup_base_integrals_gen(X2, gen.fn, gen.user, system['N'], gen.n_ri, stored=stored,
+                                              context=ctx, keep=gen))
+    return tuple(out)

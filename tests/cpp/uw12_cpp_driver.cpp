@@ -3,7 +3,9 @@
 // set up W, V, WV integrals and the ABS+ projectors, then call uw12_energy and
 // uw12_fock and check that both give the same energy.
 //
-// Usage: uw12_cpp_driver <dir>
+// Usage: uw12_cpp_driver <dir> [static]
+//   static: keep the integral and projector handles in static storage, which is
+//   destroyed after the header's function-local context at exit
 //   <dir>/dims.txt : n_ao n_df n_ri n_orb n_spin n_occ_a n_occ_b n_act_a n_act_b k0 k1
 //   <dir>/{W3,W2,W3ri,V3,V2,V3ri,WV3,WV2,S,C,occ}.bin : raw float64, column-major
 // Writes <dir>/energy.txt (uw12_energy, uw12_fock energies) and <dir>/fock.bin.
@@ -24,12 +26,18 @@ static std::vector<double> load(const std::string& path, size_t n) {
   return v;
 }
 
+// Handles in static storage: constructed before the library's context (a
+// function-local static created on first use), hence destroyed after it.
+static uw12::BaseIntegrals g_W, g_V, g_WV;
+static uw12::ABSProjectors g_abs;
+
 int main(int argc, char** argv) {
-  if (argc != 2) {
-    std::cerr << "usage: uw12_cpp_driver <dir>\n";
+  if (argc != 2 && !(argc == 3 && std::string(argv[2]) == "static")) {
+    std::cerr << "usage: uw12_cpp_driver <dir> [static]\n";
     return 2;
   }
   const std::string dir = argv[1];
+  const bool keep_static = argc == 3;
   try {
     std::ifstream d(dir + "/dims.txt");
     size_t n_ao, n_df, n_ri, n_orb, n_spin, noa, nob, naa, nab;
@@ -54,6 +62,12 @@ int main(int argc, char** argv) {
     const auto occ = n_spin == 1 ? uw12::setup_occupations(occv.data(), noa)
                                  : uw12::setup_occupations(occv.data(), noa, nob);
     const std::vector<size_t> n_active = n_spin == 1 ? std::vector<size_t>{naa} : std::vector<size_t>{naa, nab};
+    if (keep_static) {
+      g_W = W;
+      g_V = V;
+      g_WV = WV;
+      g_abs = abs;
+    }
 
     const double energy = uw12::uw12_energy(W, V, WV, abs, orbitals, occ, n_active, k0, k1);
     std::vector<double> fock(n_ao * n_ao * n_spin);

@@ -45,3 +45,34 @@ def test_cpp_api_matches_oracle(tmp_path, name):
     eo, Fo = oracle_eval(s, 1.0, 0.5)
     assert abs(e1 - eo['total']) <= 4e-9 and abs(e2 - eo['total']) <= 4e-9
     assert fock_error(Fs, Fo) <= 2e-8
+
+
+def test_cpp_static_handles_outlive_context(tmp_path):
+    """Handles kept in static storage are destroyed after the header's
+    function-local context at process exit; the context stays alive until the
+    last handle is gone, so the program exits cleanly."""
+    if not os.path.exists(DRIVER):
+        pytest.fail(f'{DRIVER} not built (python -m libuw12_amd.build)')
+    s = synth.make_system('c1')
+    write(str(tmp_path), s, 1.0, 0.5)
+    r = subprocess.run([DRIVER, str(tmp_path), 'static'], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.returncode, r.stderr)
+
+
+def test_python_context_closed_before_handles(gpu_ctx):
+    """Context.close() with live handles defers the context's deletion to the
+    last handle's destroy."""
+    import gc
+
+    import libuw12_amd as u
+    from helpers import gpu_eval, gpu_setup
+    s = synth.make_system('c1')
+    ctx = u.Context(0)
+    h = gpu_setup(s, ctx)
+    e_before, _ = gpu_eval(h, s, 1.0, 0.5)
+    ctx.close()
+    del h
+    gc.collect()
+    # the process's other contexts are unaffected
+    e_after, _ = gpu_eval(gpu_setup(s, gpu_ctx), s, 1.0, 0.5)
+    assert e_after['total'] == e_before['total']

@@ -176,3 +176,34 @@ def test_affine_two_level_k_maps(mode, K, kc):
     else:
         bound = fp32_bound(M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, Km)
         assert np.all(np.abs(got - ref) <= bound)
+
+
+@pytest.mark.parametrize('side', ['A', 'B'])
+@pytest.mark.parametrize('mode', [0, 2])
+def test_operand_span_past_lane_offsets(side, mode):
+    """An operand whose row (A) or column (B) offsets pass 2^28 doubles, the limit of the
+    buffer path's 32-bit lane offsets: the launcher splits the output into row / column
+    blocks launched from 64-bit base pointers.  The output map has an inner period of 30
+    rows / columns, so the blocks must also be whole periods of it."""
+    rng = np.random.default_rng(17 + mode)
+    big, K, n_small, stride = 300, 40, 50, 1 << 20
+    Abig = np.zeros(big * stride)
+    Abig.reshape(big, stride)[:, :K] = rng.standard_normal((big, K))
+    small = rng.standard_normal(K * n_small)
+    if side == 'A':
+        M, N = big, n_small
+        A, Am, Ak = Abig, M2(stride), M2(1)
+        B, Bk, Bn = small, M2(1), M2(K)
+        Cm, Cn = M2(1, 1000, 30), M2(10 * 1000)
+        C = rng.standard_normal(10 * 1000 * N)
+    else:
+        M, N = n_small, big
+        A, Am, Ak = small, M2(K), M2(1)
+        B, Bk, Bn = Abig, M2(1), M2(stride)
+        Cm, Cn = M2(1), M2(64, 10000, 30)
+        C = rng.standard_normal(10000 * (big // 30) + 64 * 30)
+    args = (M, N, K, 1, 0.75, 0.5, A, Am, Ak, 0, B, Bk, Bn, 0, C, Cm, Cn, 0)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode)
+    tol = 1e-12 if mode == 0 else 1e-5
+    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()

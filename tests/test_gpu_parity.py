@@ -47,6 +47,11 @@ RANDOM_CASES = [
     (7, 10, 12, [4, 3], [3, 1]),
     (23, 57, 41, [9], [9]),
     (23, 57, 41, [9, 6], [7, 6]),
+    # more beta than alpha orbitals (occupied and active): the fused two-spin evaluation's
+    # shared ABS-row buffers are sized for the larger spin before the first is formed
+    (7, 10, 12, [3, 4], [3, 4]),
+    (7, 10, 12, [3, 4], [1, 3]),
+    (23, 57, 41, [6, 9], [6, 9]),
 ]
 
 
