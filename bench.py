@@ -12,6 +12,14 @@ timed beside it on the same inputs, and the mixed result is gated against it:
 `accuracy_gate` reports the largest energy-term and Fock deviations and
 whether both are within 1e-6.  `--form fp64` times the plain path instead.
 
+Workloads (--config): c1, c2 (host integral arrays, the default timed
+workload c2), c3 / c4 (integrals generated on the device and stored), c5
+(n_ao = 1500: integrals generated on the device and not stored -- every
+evaluation regenerates its three-index blocks).  One c5 evaluation takes about
+half a minute, so the default run times c2 for the requested steps and then
+measures c5 at kappa = (1, 0) once per form (`c5_form`: mixed and FP64, the
+accuracy gate, its own roofline); `--c5-side skip` leaves that out.
+
 Prints one JSON line (see the keys below).  --gpus > 1 is rejected: this
 build measures one GPU.
 """
@@ -190,13 +198,159 @@ def cpu_baseline(system, k0, k1, pair_samples=(1, 4), full=False, gpu_result=Non
                 seconds_per_eval=round(t_full, 2), blas=cpu.blas_kind())
 
 
+CONFIG_DEFAULTS = {
+    # name: (kappa, integrals)  host: X3 / X3_ri arrays from the host generator;
+    # device: generated on the device and stored; streamed: generated, not stored
+    'c1': ((1.0, 0.5), 'host'),
+    'c2': ((1.0, 0.5), 'host'),
+    'c3': ((1.0, 0.0), 'device'),
+    'c4': ((1.0, 0.0), 'device'),
+    'c5': ((1.0, 0.0), 'streamed'),
+}
+
+
+def setup_config(name, integrals, ctx):
+    import libuw12_amd as u
+    from libuw12_amd import synth
+    t = time.perf_counter()
+    system = synth.make_system(name, with_integrals=integrals == 'host')
+    t_gen = time.perf_counter() - t
+    N, Nd, Nr = system['N'], system['Nd'], system['Nr']
+    print(f'[bench] {name}: n_ao={N} n_df={Nd} n_ri={Nr} n_occ={[len(x) for x in system["occ"]]} inputs in '
+          f'{t_gen:.1f}s', file=sys.stderr, flush=True)
+    t = time.perf_counter()
+    if integrals == 'host':
+        W = u.setup_base_integrals(*system['ints']['W'], n_ao=N, context=ctx)
+        V = u.setup_base_integrals(*system['ints']['V'], n_ao=N, context=ctx)
+        WV = u.setup_base_integrals(system['ints']['WV'][0], system['ints']['WV'][1], None, n_ao=N, context=ctx)
+    else:
+        W, V, WV = synth.device_integrals(system, ctx, stored=integrals == 'device')
+    P = u.setup_abs_projectors(system['S'], N, Nr, context=ctx)
+    t_setup = time.perf_counter() - t
+    print(f'[bench] {name}: device setup {t_setup:.1f}s ({integrals} integrals; kept DF directions W={W.n_kept} '
+          f'V={V.n_kept} WV={WV.n_kept})', file=sys.stderr, flush=True)
+    return system, (W, V, WV, P), t_setup
+
+
+def deviation(tr, Fr, t_ref, F_ref):
+    return {'term_error_eh': {k: tr[k] - t_ref[k] for k in TERM_KEYS},
+            'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(Fr, F_ref))}
+
+
+def measure(name, system, handles, ctx, kappa, form, steps, warmup, side_forms, side_reps, energy_only, label):
+    """Time `steps` evaluations in `form` after `warmup` untimed ones; then the side forms.
+    Returns the measurement dict (value = evaluations/s over the timed steps)."""
+    import libuw12_amd as u
+    k0, k1 = kappa
+    W, V, WV, P = handles
+    N, Nd, Nr = system['N'], system['Nd'], system['Nr']
+    occ_counts = [len(x) for x in system['occ']]
+    indirect = abs(k1) > 4 * np.finfo(float).eps
+    streamed = not getattr(W, 'stored', True)
+    prec = {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED}[form]
+
+    def evaluation(precision, fock=True):
+        return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
+                           indirect_term=indirect, calculate_fock=fock, scale_opp_spin=k0, scale_same_spin=k1,
+                           precision=precision)
+
+    for i in range(warmup):
+        evaluation(prec)
+        print(f'[bench] {label}: warmup {i + 1}/{warmup} done', file=sys.stderr, flush=True)
+    # each evaluation returns its energies and Fock to the host, so the library's stream has drained
+    # when form_fock returns: the wall clock brackets exactly the timed evaluations
+    ctx.set_profiling(True)
+    records = []
+    t0 = time.perf_counter()
+    for i in range(steps):
+        terms, F = evaluation(prec)
+        records.extend(ctx.profile())
+        print(f'[bench] {label}: step {i + 1}/{steps} done', file=sys.stderr, flush=True)
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    value = steps / elapsed
+    roof, table = roofline_from_profile(records)
+    gpu_flops = sum(r[2] for r in records) / steps
+    model = flop_model(N, Nd, Nr, occ_counts, indirect)
+
+    def timed_form(precision, fname, reps=max(1, side_reps)):
+        if reps > 1:
+            evaluation(precision)
+        t = time.perf_counter()
+        for _ in range(reps):
+            tr, Fr = evaluation(precision)
+        tp = (time.perf_counter() - t) / reps
+        print(f'[bench] {label}: {fname} form {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
+        return tr, Fr, tp
+
+    side = side_forms in ('auto', 'gate')
+    if form == 'fp64' or not side:
+        terms64, F64, t64 = terms, F, elapsed / steps
+    else:
+        terms64, F64, t64 = timed_form(u.FP64, 'FP64')
+    fp64 = {'evals_per_s': round(1.0 / t64, 6), 'ms_per_eval': round(1e3 * t64, 3),
+            'energy': terms64['total']} if side or form == 'fp64' else None
+    gate = None
+    if form != 'fp64' and side:
+        dev = deviation(terms, F, terms64, F64)
+        worst_e = max(abs(v) for v in dev['term_error_eh'].values())
+        gate = dict(dev, bar=GATE, max_energy_error_eh=worst_e,
+                    met=bool(worst_e <= GATE and dev['fock_max_abs_error'] <= GATE),
+                    speedup_vs_fp64=round(t64 * value, 3))
+        print(f'[bench] {label}: {form} vs FP64: max energy error {worst_e:.3e} Eh, Fock max-abs error '
+              f'{dev["fock_max_abs_error"]:.3e}, gate {"met" if gate["met"] else "NOT met"}',
+              file=sys.stderr, flush=True)
+    others = {}
+    for fname, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
+        if fname == form or side_forms != 'auto' or (streamed and fname == 'fp32'):
+            continue
+        tr, Fr, tp = timed_form(precision, fname)
+        others[f'{fname}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
+                                       speedup_vs_fp64=round(t64 / tp, 3))
+    if energy_only and side_forms == 'auto':
+        # the energy-only call (uw12_energy, libuw12.hpp:129-140) in the timed form
+        reps = max(1, side_reps)
+        evaluation(prec, fock=False)
+        t = time.perf_counter()
+        for _ in range(reps):
+            te = evaluation(prec, fock=False)[0]
+        tp = (time.perf_counter() - t) / reps
+        print(f'[bench] {label}: energy-only ({form}) {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
+        others['energy_only'] = {'form': form, 'evals_per_s': round(1.0 / tp, 6), 'ms_per_eval': round(1e3 * tp, 3),
+                                 'term_error_eh': {k: te[k] - terms64[k] for k in TERM_KEYS}}
+    return dict(value=value, elapsed=elapsed, terms=terms, F=F, terms64=terms64, F64=F64, roofline=roof,
+                table=table, gate=gate, fp64=fp64, others=others, gpu_flops=gpu_flops, model=model,
+                occ_counts=occ_counts)
+
+
+FORM_TEXT = {'fp64': 'plain FP64',
+             'fp32': 'FP32 form (every contraction on FP32 matrix cores, FP64 storage and reductions)',
+             'mixed': 'mixed form (FP32 matrix cores for the Fock intermediates K~, L, R, Xi, G, X_i/Y_i; FP64 '
+                      'for the orbital transforms, the integral generation products, energies and Fock '
+                      'accumulation), gated at <=1e-6 Eh per term and <=1e-6 Fock max-abs against the plain FP64 '
+                      'path of the same run'}
+INTEGRALS_TEXT = {'host': 'host integral arrays copied to HBM',
+                  'device': 'integrals generated on the device, stored in HBM',
+                  'streamed': 'integrals generated on the device, not stored: every evaluation regenerates its '
+                              'three-index blocks (the rotated W tensor is kept between evaluations)'}
+
+
+def workload(name, system, kappa, form, integrals):
+    N, Nd, Nr = system['N'], system['Nd'], system['Nr']
+    occ_counts = [len(x) for x in system['occ']]
+    return (f'{name}: UW12 energy+Fock, n_ao={N} n_df={Nd} n_ri={Nr} n_occ={occ_counts}, kappa=({kappa[0]},'
+            f'{kappa[1]}), {INTEGRALS_TEXT[integrals]}, {FORM_TEXT[form]}')
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=5)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--config', default='c2', choices=['c1', 'c2', 'c3', 'c4'])
-    ap.add_argument('--kappa', default='1.0,0.5')
+    ap.add_argument('--config', default='c2', choices=sorted(CONFIG_DEFAULTS))
+    ap.add_argument('--kappa', default=None, help='k0,k1 (default: (1,0.5) for c1/c2, (1,0) for c3-c5)')
+    ap.add_argument('--integrals', default=None, choices=['host', 'device', 'streamed'],
+                    help='default: host for c1/c2, device for c3/c4, streamed for c5')
     ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32'],
                     help='precision form of the timed steps (the plain FP64 path is run beside it)')
     ap.add_argument('--side-reps', type=int, default=3,
@@ -205,6 +359,8 @@ def main():
                     help='gate: only the FP64 evaluation the accuracy gate needs; skip: no FP64 / FP32 / mixed '
                          'evaluations beside the timed ones (no gate), for profiler runs that should see only the '
                          'timed form')
+    ap.add_argument('--c5-side', default='auto', choices=['auto', 'skip'],
+                    help='auto: after the timed c2 steps, one c5 evaluation per form (mixed timed, FP64 gate)')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
                     help='auto: sampled oracle timing; full: one complete oracle evaluation, with parity report')
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
@@ -227,147 +383,35 @@ def main():
         print(f'bench.py measures one GPU; --gpus {args.gpus} is not supported by this build '
               '(the multi-GPU design shards the DF index and is documented, not run).', file=sys.stderr)
         sys.exit(2)
-    k0, k1 = (float(x) for x in args.kappa.split(','))
+    kappa_default, integrals_default = CONFIG_DEFAULTS[args.config]
+    kappa = tuple(float(x) for x in args.kappa.split(',')) if args.kappa else kappa_default
+    integrals = args.integrals or integrals_default
+    if integrals == 'streamed' and args.form == 'fp32':
+        print('the FP32 form needs stored integrals (use --integrals device)', file=sys.stderr)
+        sys.exit(2)
 
     import libuw12_amd as u
-    from libuw12_amd import synth
 
-    # the larger configurations stream their integrals per DF shell (ThreeIndexFn form)
-    # instead of holding X3 / X3_ri on the host
-    stream = args.config in ('c3', 'c4')
-    t = time.perf_counter()
-    system = synth.make_system(args.config, with_integrals=not stream)
-    t_gen = time.perf_counter() - t
-    N, Nd, Nr = system['N'], system['Nd'], system['Nr']
-    occ_counts = [len(x) for x in system['occ']]
-    print(f'[bench] {args.config}: n_ao={N} n_df={Nd} n_ri={Nr} n_occ={occ_counts} generated in {t_gen:.1f}s',
-          file=sys.stderr, flush=True)
     ctx = u.Context(0)
     if args.verbose:
         ctx.set_verbosity(1)
-    t = time.perf_counter()
-    if stream:
-        handles = []
-        for kind in ('W', 'V', 'WV'):
-            X2, sizes, fn, fn_ri = synth.shell_integrals(system, kind)
-            handles.append(u.setup_base_integrals_fn(X2, fn, fn_ri, sizes, N, Nr if fn_ri else 0, context=ctx))
-            print(f'[bench] {kind} integrals streamed ({len(sizes)} DF shells)', file=sys.stderr, flush=True)
-        W, V, WV = handles
-    else:
-        W = u.setup_base_integrals(*system['ints']['W'], n_ao=N, context=ctx)
-        V = u.setup_base_integrals(*system['ints']['V'], n_ao=N, context=ctx)
-        WV = u.setup_base_integrals(system['ints']['WV'][0], system['ints']['WV'][1], None, n_ao=N, context=ctx)
-    P = u.setup_abs_projectors(system['S'], N, Nr, context=ctx)
-    t_setup = time.perf_counter() - t
-    print(f'[bench] device setup {t_setup:.1f}s (kept DF directions W={W.n_kept} V={V.n_kept} WV={WV.n_kept})',
-          file=sys.stderr, flush=True)
-    indirect = abs(k1) > 4 * np.finfo(float).eps
-    prec = {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED}[args.form]
-
-    def evaluation(precision):
-        return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
-                           indirect_term=indirect, calculate_fock=True, scale_opp_spin=k0, scale_same_spin=k1,
-                           precision=precision)
-
-    for i in range(args.warmup):
-        evaluation(prec)
-        print(f'[bench] warmup {i + 1}/{args.warmup} done', file=sys.stderr, flush=True)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None  # noqa: E731
-    ctx.set_profiling(True)
-    records = []
-    sync()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        terms, F = evaluation(prec)
-        records.extend(ctx.profile())
-        print(f'[bench] step {i + 1}/{args.steps} done', file=sys.stderr, flush=True)
-    sync()
-    elapsed = time.perf_counter() - t0
-    ctx.set_profiling(False)
-    value = args.steps / elapsed
-    roof, table = roofline_from_profile(records)
+    system, handles, t_setup = setup_config(args.config, integrals, ctx)
+    m = measure(args.config, system, handles, ctx, kappa, args.form, args.steps, args.warmup, args.side_forms,
+                args.side_reps, energy_only=integrals != 'streamed', label=args.config)
+    roof = m['roofline']
     roof['traffic'] = measured_traffic(roof['kernel'], args.config, args.form)
     if roof['traffic'] is not None:
         roof['traffic_source'] = f'{traffic_profile(args.config, args.form)} (rocprofv3 FETCH_SIZE + WRITE_SIZE)'
-    gpu_flops = sum(r[2] for r in records) / args.steps
-    model = flop_model(N, Nd, Nr, occ_counts, indirect)
     if args.profile_table:
-        print(json.dumps(table, indent=1), file=sys.stderr)
-
-    def deviation(tr, Fr, t_ref, F_ref):
-        return {'term_error_eh': {k: tr[k] - t_ref[k] for k in TERM_KEYS},
-                'fock_max_abs_error': max(float(np.abs(a - b).max()) for a, b in zip(Fr, F_ref))}
-
-    def timed_form(precision, name, reps=max(1, args.side_reps)):
-        if reps > 1:
-            evaluation(precision)
-        sync()
-        t = time.perf_counter()
-        for _ in range(reps):
-            tr, Fr = evaluation(precision)
-        sync()
-        tp = (time.perf_counter() - t) / reps
-        print(f'[bench] {name} form: {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
-        return tr, Fr, tp
-
-    # the plain FP64 path on the same inputs: reference for the reduced-precision forms
-    side = args.side_forms in ('auto', 'gate')
-    if args.form == 'fp64' or not side:
-        terms64, F64, t64 = terms, F, elapsed / args.steps
-    else:
-        terms64, F64, t64 = timed_form(u.FP64, 'FP64')
-    fp64 = {'evals_per_s': round(1.0 / t64, 6), 'ms_per_eval': round(1e3 * t64, 3),
-            'energy': terms64['total']} if side or args.form == 'fp64' else None
-    gate = None
-    if args.form != 'fp64' and side:
-        dev = deviation(terms, F, terms64, F64)
-        worst_e = max(abs(v) for v in dev['term_error_eh'].values())
-        gate = dict(dev, bar=GATE, max_energy_error_eh=worst_e,
-                    met=bool(worst_e <= GATE and dev['fock_max_abs_error'] <= GATE),
-                    speedup_vs_fp64=round(t64 * value, 3))
-        print(f'[bench] {args.form} vs FP64: max energy error {worst_e:.3e} Eh, Fock max-abs error '
-              f'{dev["fock_max_abs_error"]:.3e}, gate {"met" if gate["met"] else "NOT met"}',
-              file=sys.stderr, flush=True)
-    others = {}
-    for name, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
-        if name == args.form or args.side_forms != 'auto':
-            continue
-        tr, Fr, tp = timed_form(precision, name)
-        others[f'{name}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
-                                      speedup_vs_fp64=round(t64 / tp, 3))
-    if args.side_forms == 'auto':
-        # the energy-only call (uw12_energy, libuw12.hpp:129-140) in the timed form: no Fock, the
-        # same-spin 3e energy from its own FP64 products instead of the Fock's L
-        def energy_only():
-            return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
-                               indirect_term=indirect, calculate_fock=False, scale_opp_spin=k0,
-                               scale_same_spin=k1, precision=prec)[0]
-        reps = max(1, args.side_reps)
-        energy_only()
-        sync()
-        t = time.perf_counter()
-        for _ in range(reps):
-            te = energy_only()
-        sync()
-        tp = (time.perf_counter() - t) / reps
-        print(f'[bench] energy-only ({args.form}): {1e3 * tp:.1f} ms per evaluation', file=sys.stderr, flush=True)
-        others['energy_only'] = {'form': args.form, 'evals_per_s': round(1.0 / tp, 6), 'ms_per_eval': round(1e3 * tp, 3),
-                                 'term_error_eh': {k: te[k] - terms64[k] for k in TERM_KEYS}}
+        print(json.dumps(m['table'], indent=1), file=sys.stderr)
     cpu = None
-    if args.cpu_baseline in ('auto', 'full') and not stream:
+    if args.cpu_baseline in ('auto', 'full') and integrals == 'host':
         print('[bench] timing CPU baseline (oracle)', file=sys.stderr, flush=True)
-        cpu = cpu_baseline(system, k0, k1, full=args.cpu_baseline == 'full',
-                           gpu_result={'fp64': (terms64, F64), args.form: (terms, F)} if side
-                           else {args.form: (terms, F)})
-    form_text = {'fp64': 'plain FP64',
-                 'fp32': 'FP32 form (every contraction on FP32 matrix cores, FP64 storage and reductions)',
-                 'mixed': 'mixed form (FP32 matrix cores for the Fock intermediates K~, L, R, Xi, G, X_i/Y_i; FP64 '
-                          'for the orbital transforms, energies and Fock accumulation), gated at <=1e-6 Eh per '
-                          'term and <=1e-6 Fock max-abs against the plain FP64 path of the same run'}[args.form]
+        side = args.side_forms in ('auto', 'gate')
+        cpu = cpu_baseline(system, kappa[0], kappa[1], full=args.cpu_baseline == 'full',
+                           gpu_result={'fp64': (m['terms64'], m['F64']), args.form: (m['terms'], m['F'])} if side
+                           else {args.form: (m['terms'], m['F'])})
+    value = m['value']
     out = {
         'metric': METRIC,
         'value': round(value, 6),
@@ -375,26 +419,43 @@ def main():
         'n_gpus': 1,
         'steps': args.steps,
         'warmup': args.warmup,
-        'ms_per_step': round(1e3 * elapsed / args.steps, 3),
+        'ms_per_step': round(1e3 * m['elapsed'] / args.steps, 3),
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': {'fp64': 'f64', 'fp32': 'f32', 'mixed': 'f32+f64'}[args.form],
         'data': 'synthetic (seeded s-Gaussian cluster, analytic Boys-F0 integrals)',
-        'config': {'workload': f'{args.config}: UW12 energy+Fock, n_ao={N} n_df={Nd} n_ri={Nr} n_occ={occ_counts}, '
-                               f'kappa=({k0},{k1}), {form_text}',
-                   'form': args.form, 'kappa': [k0, k1], 'parallelism': 'single-gpu'},
+        'config': {'workload': workload(args.config, system, kappa, args.form, integrals),
+                   'form': args.form, 'kappa': list(kappa), 'integrals': integrals, 'parallelism': 'single-gpu'},
         'roofline': roof,
         'cpu_baseline': cpu,
-        'accuracy_gate': gate,
-        'fp64_form': fp64,
-        'energy': terms['total'],
-        'gpu_tflops_executed': round(gpu_flops * value / 1e12, 3),
-        'model_flops_per_eval': model,
-        'executed_flops_per_eval': gpu_flops,
+        'accuracy_gate': m['gate'],
+        'fp64_form': m['fp64'],
+        'energy': m['terms']['total'],
+        'gpu_tflops_executed': round(m['gpu_flops'] * value / 1e12, 3),
+        'model_flops_per_eval': m['model'],
+        'executed_flops_per_eval': m['gpu_flops'],
         'setup_s': round(t_setup, 2),
     }
-    out.update(others)
+    out.update(m['others'])
+    if args.config == 'c2' and args.c5_side == 'auto':
+        # the n_ao = 1500 configuration the metric is quoted on: one timed evaluation per form
+        del handles, m
+        ctx.close()
+        ctx = u.Context(0)
+        if args.verbose:
+            ctx.set_verbosity(1)
+        k5, i5 = CONFIG_DEFAULTS['c5']
+        sys5, h5, t5 = setup_config('c5', i5, ctx)
+        m5 = measure('c5', sys5, h5, ctx, k5, 'mixed', 1, 1, 'gate', 1, energy_only=False, label='c5')
+        r5 = m5['roofline']
+        r5['traffic'] = measured_traffic(r5['kernel'], 'c5', 'mixed')
+        out['c5_form'] = {
+            'workload': workload('c5', sys5, k5, 'mixed', i5),
+            'evals_per_s': round(m5['value'], 6), 'ms_per_eval': round(1e3 * m5['elapsed'], 3),
+            'steps': 1, 'warmup': 1, 'fp64_form': m5['fp64'], 'accuracy_gate': m5['gate'], 'roofline': r5,
+            'energy': m5['terms']['total'], 'executed_flops_per_eval': m5['gpu_flops'],
+            'model_flops_per_eval': m5['model'], 'setup_s': round(t5, 2)}
     print(json.dumps(out), flush=True)
 
 

@@ -686,6 +686,10 @@ static void check_desc(const GemmDesc& d) {
 
 namespace {
 
+// Largest split-K partial-sum buffer (doubles) a launch may ask for; past it
+// the FP32 k chunks run as consecutive accumulating launches instead.
+constexpr int64_t kSplitWsMax = int64_t(1) << 28;
+
 // One launch over k in [kbase, d.K): split over K when the grid would leave
 // most of the 256 CUs idle (kbase == 0 only).
 void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hipStream_t stream, Workspace* ws,
@@ -726,8 +730,20 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
     splits = choose_splits(wgs, kt, klen, d.batch, slots_for(BM, BN, akc, bkc, d.ks != nullptr, d.prec));
   if (d.prec == 1 && klen > kF32Chunk) {
     // FP32 accumulation runs over at most kF32Chunk k values; chunks add up in FP64
-    if (kbase != 0 || !ws) throw Error(-2, "gemm: FP32 k remainder longer than one chunk");
-    splits = std::max(splits, (klen + kF32Chunk - 1) / kF32Chunk);
+    const int nchunk = (klen + kF32Chunk - 1) / kF32Chunk;
+    if (kbase != 0 || !ws || int64_t(std::max(splits, nchunk)) * d.batch * d.M * d.N > kSplitWsMax) {
+      // large outputs (or a k remainder launch): the chunks run one after another,
+      // each adding its FP32 sum onto the FP64 output (beta = 1 after the first),
+      // so no partial-sum buffer of the output's size per chunk is needed
+      for (int c = 0; c < nchunk; ++c) {
+        GemmDesc e = d;
+        e.K = std::min(d.K, kbase + (c + 1) * kF32Chunk);
+        if (c > 0) e.beta = 1.0;
+        launch_gemm(e, kbase + c * kF32Chunk, fast, akc, bkc, stream, nullptr, prof);
+      }
+      return;
+    }
+    splits = std::max(splits, nchunk);
     if (splits > 65535) throw Error(-2, "gemm: too many FP32 k chunks");
   }
   p.kchunk = ((kt + splits - 1) / splits) * BK;

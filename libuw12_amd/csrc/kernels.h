@@ -52,7 +52,7 @@ void scale_cols(const double* in, int64_t rows, int64_t cols, const double* sc, 
 // out = (in diag(sc))^T: in is rows x cols (column-major), out is cols x rows.
 void transpose_scale(const double* in, int64_t rows, int64_t cols, const double* sc, double* out, hipStream_t s);
 // out[m + rows (i + ni b)] = d[m] cb[b] in[m + si i + sb b] for m < rows, i < ni, b < nb
-// (cb == nullptr: 1).
+// (cb == nullptr: 1).  out may be in itself when si = rows and sb = rows ni.
 void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t si, int64_t sb, const double* d,
                 const double* cb, double* out, hipStream_t s);
 // Columns of one block of occupied pairs (j, k), j in [j0, j0 + nj) active, k in

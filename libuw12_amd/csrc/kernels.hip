@@ -336,10 +336,11 @@ __global__ void scale_cols_kernel(const double* __restrict__ in, int64_t rows, i
 
 // One block per output column (i, b), the threads along the rows: reads and
 // writes are both contiguous runs of `rows` doubles.
-__global__ __launch_bounds__(256) void scale_rows_kernel(const double* __restrict__ in, int64_t rows, int64_t ni,
-                                                         int64_t ncol, int64_t si, int64_t sb,
-                                                         const double* __restrict__ d, const double* __restrict__ cb,
-                                                         double* __restrict__ out) {
+// in and out may be the same array (in place: each element is read, then written,
+// by one thread).
+__global__ __launch_bounds__(256) void scale_rows_kernel(const double* in, int64_t rows, int64_t ni, int64_t ncol,
+                                                         int64_t si, int64_t sb, const double* __restrict__ d,
+                                                         const double* __restrict__ cb, double* out) {
   for (int64_t col = blockIdx.x; col < ncol; col += gridDim.x) {
     const int64_t i = col % ni, b = col / ni;
     const double c = cb ? cb[b] : 1.0;

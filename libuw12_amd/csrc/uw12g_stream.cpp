@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <string>
 #include <vector>
 
@@ -55,10 +56,16 @@ void run_gemm64(uw12g_context* ctx, const GemmDesc& d, const char* stage) {
   ctx->prec = prec;
 }
 
+// One block of three-index integrals from the integrals' device generator
+// (timed as the "three_index_generator" kernel when profiling is on).
 void generate(uw12g_context* ctx, const uw12g_base_integrals* X, int B0, int n, bool with_ri, double* out) {
-  const int rc = X->gen(X->gen_user, size_t(B0), size_t(n), with_ri ? 1 : 0, out, ctx->stream);
-  require(rc == 0, "three-index generator failed for DF functions " + std::to_string(B0) + ".." +
-                       std::to_string(B0 + n), -6);
+  const double rows = double(X->N) + (with_ri ? double(X->Nr) : 0.0);
+  timed(ctx, with_ri ? "st_generate_abs" : "st_generate_ao", "three_index_generator", 0.0, 8.0 * rows * X->N * n,
+        [&] {
+          const int rc = X->gen(X->gen_user, size_t(B0), size_t(n), with_ri ? 1 : 0, out, ctx->stream);
+          require(rc == 0, "three-index generator failed for DF functions " + std::to_string(B0) + ".." +
+                               std::to_string(B0 + n), -6);
+        });
   UW12G_HIP(hipGetLastError());
 }
 
@@ -90,28 +97,18 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
                                hipMemcpyDeviceToDevice, st));
   auto Ca = [&](int s) { return Coall + size_t(N) * (off[s] + sd[s].c); };
 
-  // ---- block sizes: DF blocks of the AO-only passes and of pass 2, ABS rows per pass-2 block
-  size_t free_b = 0, total_b = 0;
-  UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
-  const int64_t cap = int64_t(ctx->block_cap);
+  size_t free0 = 0, total_b = 0;
+  UW12G_HIP(hipMemGetInfo(&free0, &total_b));
+  // ---- block sizes follow the free device memory (the big buffers of an earlier
+  // streamed evaluation were released at its end); a block cap set through the
+  // test hook also bounds them (tiny caps force many blocks)
+  const bool capped = ctx->block_cap != ctx->block_cap_default;
+  const int64_t cap = capped ? int64_t(ctx->block_cap) : INT64_MAX / 4;
   int amax = 0;
   for (int s = 0; s < ns; ++s) amax = std::max(amax, sd[s].a);
   const int NdMax = std::max(std::max(NdW, NdV), NdWV);
-  const int bs_ao = int(std::max<int64_t>(1, std::min<int64_t>(NdMax, cap / std::max<int64_t>(int64_t(NN), 1))));
-  // persistent: Phi_W, Phi_V, (ik|A) of both operators, t, T~ and X per spin
-  int64_t persist = int64_t(N) * O * (DW + DV);
-  for (int s = 0; s < ns; ++s) persist += int64_t(sd[s].a) * sd[s].o * (DW + DV) + 3 * int64_t(DW) * DV;
-  const int64_t avail = int64_t(double(free_b) * 0.85 / 8.0) - persist - int64_t(NN) * bs_ao -
-                        int64_t(N) * O * bs_ao;
-  int bs = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(std::max(NdW, NdV)), 256,
-                                                       cap / std::max<int64_t>(int64_t(M) * N, 1)})));
-  // per ABS row of a pass-2 block: Gh_W, Gh_V, Xi, H, Psi_W, Psi_V, diag(1/s) Psi_V
-  const int64_t per_row = int64_t(N) * (DW + DV + DW + bs) + int64_t(amax) * (DW + 2 * DV);
-  int64_t bm = (avail - int64_t(M) * N * bs) / std::max<int64_t>(per_row, 1);
-  bm = std::min<int64_t>(bm, std::max<int64_t>(16, cap / std::max<int64_t>(int64_t(N) * std::max(DV, 1), 1)));
-  bm = std::min<int64_t>(bm, Mr);
-  if (bm >= 16) bm = bm / 16 * 16;  // whole 16-deep slabs of the (m, A) sums
-  require(bm >= 1 || Mr == 0, "not enough device memory for one block of the ABS rows", -5);
+  const int bs_ao = int(std::max<int64_t>(
+      1, std::min<int64_t>({int64_t(NdMax), 1024, cap / std::max<int64_t>(int64_t(NN), 1)})));
 
   // ================= pass 0: two-electron direct term =================
   {
@@ -295,6 +292,8 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
     if (g.K > 0) run_gemm(ctx, g, "4e_tab", true);
     else fill(t[s], int64_t(DW) * DV, 0.0, st);
   }
+  ctx->release("st_o2W");
+  ctx->release("st_o2V");
   for (int s = 0; s < ns; ++s)
     for (int u = 0; u < ns; ++u) {
       const double f = spin_factor(ns, s, u, k0, k1);
@@ -320,26 +319,41 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
   progress(ctx, "streamed: AO-row transforms, t and the 4e direct energy");
 
   // ================= pass 2: ABS rows, blocked over S's kept eigenvectors =================
+  ctx->release("st_Gao");
+  ctx->release("st_phi0");
   if (Mr > 0 && DW > 0 && DV > 0 && O > 0) {
-    const int bmx = int(bm);
-    double* Gfull = ctx->buf("st_G0", size_t(M) * N * bs);
-    double* H = ctx->buf("st_H", size_t(bmx) * N * bs);
-    double* Gh[2] = {ctx->buf("st_GhW", size_t(bmx) * N * DW), ctx->buf("st_GhV", size_t(bmx) * N * DV)};
-    double* psiW = ctx->buf("st_psiW", size_t(bmx) * std::max(amax, 1) * DW);
-    double* psiV = ctx->buf("st_psiV", size_t(bmx) * std::max(amax, 1) * DV);
-    double* psiVt = ctx->buf("st_psiVt", size_t(bmx) * std::max(amax, 1) * DV);
-    double* Xi = calc_fock ? ctx->buf("st_Xi", size_t(bmx) * N * DW) : nullptr;
-    int nblk = 0;
-    for (int m0 = 0; m0 < Mr; m0 += bmx, ++nblk) {
-      const int bmm = std::min(bmx, Mr - m0);
+    const int NdX = std::max(NdW, NdV);
+    // generator calls of bs DF functions; the metric product (.) T runs over
+    // super-blocks of bsup of them (fewer read-modify-write passes over Gh)
+    const int bs = int(std::max<int64_t>(1, std::min<int64_t>({int64_t(NdX), 256, cap / std::max<int64_t>(int64_t(M) * N, 1)})));
+    const int bsup = int(std::max<int64_t>(bs, std::min<int64_t>(NdX, int64_t(4) * bs) / bs * bs));
+    // W's rotated tensor Gh_W = U^T Gamma_W (r x N x D_W, the operator with fewer kept DF
+    // directions on the recipe) is kept on the integrals across evaluations when it fits in
+    // half of the device memory: pass 2 then regenerates only V
+    const int64_t ghw_n = int64_t(Mr) * N * DW;
+    const bool cache_w = capped ? ghw_n <= cap : ghw_n * 8 <= int64_t(total_b / 2);
+    size_t free_b = 0, total_b2 = 0;
+    auto plan_rows = [&](int64_t per_row, int64_t fixed) {
+      UW12G_HIP(hipMemGetInfo(&free_b, &total_b2));
+      int64_t rows = (int64_t(double(free_b) * 0.85 / 8.0) - fixed) / std::max<int64_t>(per_row, 1);
+      rows = std::min<int64_t>(rows, Mr);
+      if (capped)
+        rows = std::min<int64_t>(rows, std::max<int64_t>(16, cap / std::max<int64_t>(int64_t(N) * std::max(DV, 1), 1)));
+      if (rows >= 16) rows = rows / 16 * 16;  // whole 16-deep slabs of the (m, A) sums
+      require(rows >= 1, "not enough device memory for one block of the ABS rows", -5);
+      return int(rows);
+    };
+    // Gh[m, nu, A] for ABS rows [m0, m0 + bmm) of X: out[m + ldm nu + ldmn A]
+    auto form_gh = [&](const uw12g_base_integrals* X, int m0, int bmm, double* out, int64_t ldm, int64_t ldmn,
+                       double* Gfull, double* H) {
+      const int Nd = X->Nd, D = X->D;
       const int64_t bN = int64_t(bmm) * N;
-      for (int op = 0; op < 2; ++op) {
-        const uw12g_base_integrals* X = op == 0 ? W : V;
-        const int Nd = X->Nd, D = X->D;
-        for (int B0 = 0; B0 < Nd; B0 += bs) {
-          const int bn = std::min(bs, Nd - B0);
-          generate(ctx, X, B0, bn, true, Gfull);
-          GemmDesc g;  // H[m, nu, j] = sum_mu' U[mu', m0 + m] G0[mu', nu, B0 + j]
+      for (int B0 = 0; B0 < Nd; B0 += bsup) {
+        const int nsup = std::min(bsup, Nd - B0);
+        for (int b0 = B0; b0 < B0 + nsup; b0 += bs) {
+          const int bn = std::min(bs, B0 + nsup - b0);
+          generate(ctx, X, b0, bn, true, Gfull);
+          GemmDesc g;  // H[m, nu, j] = sum_mu' U[mu', m0 + m] G0[mu', nu, b0 + j]
           g.M = bmm;
           g.N = N;
           g.K = M;
@@ -351,28 +365,61 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
           g.Bk = lin(1);
           g.Bn = lin(M);
           g.Bb = int64_t(M) * N;
-          g.C = H;
+          g.C = H + bN * (b0 - B0);
           g.Cm = lin(1);
           g.Cn = lin(bmm);
           g.Cb = bN;
           run_gemm64(ctx, g, "st_abs_rotate");
-          GemmDesc h;  // Gh[(m, nu), A] += sum_j H[(m, nu), j] T[B0 + j, A]
-          h.M = int(bN);
-          h.N = D;
-          h.K = bn;
-          h.beta = B0 == 0 ? 0.0 : 1.0;
-          h.A = H;
-          h.Am = lin(1);
-          h.Ak = lin(bN);
-          h.B = X->T.p + B0;
-          h.Bk = lin(1);
-          h.Bn = lin(Nd);
-          h.C = Gh[op];
-          h.Cm = lin(1);
-          h.Cn = lin(bN);
-          run_gemm64(ctx, h, "st_gh_metric");
         }
+        GemmDesc h;  // Gh[(m, nu), A] += sum_j H[(m, nu), j] T[B0 + j, A]
+        h.M = int(bN);
+        h.N = D;
+        h.K = nsup;
+        h.beta = B0 == 0 ? 0.0 : 1.0;
+        h.A = H;
+        h.Am = lin(1);
+        h.Ak = lin(bN);
+        h.B = X->T.p + B0;
+        h.Bk = lin(1);
+        h.Bn = lin(Nd);
+        h.C = out;
+        h.Cm = ldm == bmm ? lin(1) : two(uint32_t(bmm), 1, ldm);
+        h.Cn = lin(ldmn);
+        run_gemm64(ctx, h, "st_gh_metric");
       }
+    };
+    if (cache_w && W->gt_id != P->id) {
+      W->gt_id = 0;
+      W->gt.reset();
+      W->gt.get(size_t(ghw_n));
+      const int bmb = plan_rows(int64_t(N) * bsup, int64_t(M) * N * bs);
+      double* Gfull = ctx->buf("st_G0", size_t(M) * N * bs);
+      double* H = ctx->buf("st_H", size_t(bmb) * N * bsup);
+      for (int m0 = 0; m0 < Mr; m0 += bmb) form_gh(W, m0, std::min(bmb, Mr - m0), W->gt.p + m0, Mr, int64_t(Mr) * N,
+                                                  Gfull, H);
+      ctx->release("st_H");
+      W->gt_id = P->id;
+      progress(ctx, "streamed: rotated W integrals kept (%.1f GB)", ghw_n * 8.0 / 1e9);
+    }
+    // per ABS row of a block: Gh_V, Xi, H, Psi_W, diag(1/s) Psi_V (and Gh_W when not kept)
+    const int64_t per_row = int64_t(N) * (DV + (calc_fock ? DW : 0) + bsup + (cache_w ? 0 : DW)) +
+                            int64_t(amax) * (DW + DV);
+    const int bmx = plan_rows(per_row, int64_t(M) * N * bs);
+    double* Gfull = ctx->buf("st_G0", size_t(M) * N * bs);
+    double* H = ctx->buf("st_H", size_t(bmx) * N * bsup);
+    double* GhW = cache_w ? nullptr : ctx->buf("st_GhW", size_t(bmx) * N * DW);
+    double* GhV = ctx->buf("st_GhV", size_t(bmx) * N * DV);
+    double* psiW = ctx->buf("st_psiW", size_t(bmx) * std::max(amax, 1) * DW);
+    double* psiV = ctx->buf("st_psiV", size_t(bmx) * std::max(amax, 1) * DV);
+    double* Xi = calc_fock ? ctx->buf("st_Xi", size_t(bmx) * N * DW) : nullptr;
+    for (int m0 = 0; m0 < Mr; m0 += bmx) {
+      const int bmm = std::min(bmx, Mr - m0);
+      const int64_t bN = int64_t(bmm) * N;
+      // Gh_W of the block: a slice of the kept tensor, or formed here
+      const double* ghw = cache_w ? W->gt.p + m0 : GhW;
+      const int64_t wm = cache_w ? Mr : bmm, wmn = cache_w ? int64_t(Mr) * N : bN;
+      if (!cache_w) form_gh(W, m0, bmm, GhW, bmm, bN, Gfull, H);
+      form_gh(V, m0, bmm, GhV, bmm, bN, Gfull, H);
       for (int s = 0; s < ns; ++s) {
         const int a = sd[s].a;
         if (a == 0) {
@@ -384,10 +431,10 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
             g.N = a;
             g.K = N;
             g.batch = op == 0 ? DW : DV;
-            g.A = Gh[op];
+            g.A = op == 0 ? ghw : GhV;
             g.Am = lin(1);
-            g.Ak = lin(bmm);
-            g.Ab = bN;
+            g.Ak = lin(op == 0 ? wm : bmm);
+            g.Ab = op == 0 ? wmn : bN;
             g.B = Ca(s);
             g.Bk = lin(1);
             g.Bn = lin(N);
@@ -397,7 +444,8 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
             g.Cb = int64_t(bmm) * a;
             run_gemm(ctx, g, "psi_transform", true);
           }
-          scale_rows(psiV, bmm, a, DV, bmm, int64_t(bmm) * a, P->dinv.p + m0, nullptr, psiVt, st);
+          // diag(1/s) Psi_V in place
+          scale_rows(psiV, bmm, a, DV, bmm, int64_t(bmm) * a, P->dinv.p + m0, nullptr, psiV, st);
           GemmDesc xg;  // X[A, B] += sum_{m, i} Psi_W[m, i, A] (1/s_m) Psi_V[m, i, B]  (direct_utils.cpp:62-133)
           xg.M = DW;
           xg.N = DV;
@@ -406,7 +454,7 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
           xg.A = psiW;
           xg.Am = lin(int64_t(bmm) * a);
           xg.Ak = lin(1);
-          xg.B = psiVt;
+          xg.B = psiV;
           xg.Bk = lin(1);
           xg.Bn = lin(int64_t(bmm) * a);
           xg.C = x[s];
@@ -419,7 +467,7 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
         a1.M = int(bN);
         a1.N = DW;
         a1.K = DV;
-        a1.A = Gh[1];
+        a1.A = GhV;
         a1.Am = lin(1);
         a1.Ak = lin(bN);
         a1.B = Tb[s];
@@ -435,9 +483,9 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
         a3.K = bmm * DW;
         a3.alpha = -1.0;
         a3.beta = 1.0;
-        a3.A = Gh[0];
-        a3.Am = lin(bmm);
-        a3.Ak = two(uint32_t(bmm), 1, bN);
+        a3.A = ghw;
+        a3.Am = lin(wm);
+        a3.Ak = two(uint32_t(bmm), 1, wmn);
         a3.ks = P->dinv.p + m0;
         a3.Km = two(uint32_t(bmm), 1, 0);
         a3.B = Xi;
@@ -450,7 +498,7 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
       }
       progress(ctx, "streamed: ABS rows %d-%d of %d", m0, m0 + bmm, Mr);
     }
-    (void)nblk;
+    for (const char* nm : {"st_G0", "st_H", "st_GhW", "st_GhV", "st_psiW", "st_psiV", "st_Xi"}) ctx->release(nm);
   } else {
     for (int s = 0; s < ns; ++s) fill(x[s], int64_t(DW) * DV, 0.0, st);
   }
@@ -470,7 +518,11 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
       dd.y = t[u];
       dot(dd, -f, Eacc + UW12G_E3_DIRECT, 1, partial, st);
     }
-  if (!calc_fock) return;
+  if (!calc_fock) {
+    ctx->release("st_phiW");
+    ctx->release("st_phiV");
+    return;
+  }
 
   // X dt~ and 4e direct Fock: F += Phi_V (w_k Phi_W Z)^T, Z = T~ - sW sV sum_u c_u X_u
   // (four_electron.cpp:11-77, direct_utils.cpp:164-204)
@@ -513,6 +565,7 @@ void evaluate_streamed(uw12g_context* ctx, const uw12g_base_integrals* W, const 
     run_gemm(ctx, f, "direct_fock_F", true);
   }
   progress(ctx, "streamed: direct Fock");
+  for (const char* nm : {"st_phiW", "st_phiV", "G"}) ctx->release(nm);
 }
 
 }  // namespace uw12g
