@@ -70,16 +70,19 @@ def abs_pinv(S):
     return _es('pm,m,qm->pq', Vk, 1 / sig[keep].astype(LD), Vk)
 
 
-def energy_terms(ints, S, C, occ, n_active, k0, k1, indirect=True):
-    """Six energy pieces and their total as Python floats (rounded from long double).
+def energy_terms(ints, S, C, occ, n_active, k0, k1, indirect=True, raw=False):
+    """Six energy pieces and their total as Python floats (rounded from long double),
+    or as numpy.longdouble values with raw=True.
 
     Same arguments as ``brute.energy_terms``: ints {'W', 'V', 'WV': (X3, X2, X3ri)},
-    C / occ / n_active per spin."""
+    C / occ / n_active per spin; C may be given in long double (finite differences
+    along orbital paths then carry no FP64 rounding of the perturbed orbitals)."""
     n_spin = len(C)
     N = C[0].shape[0]
     g = 2.0 if n_spin == 1 else 1.0
     e = g
-    Co = [(np.asarray(C[s])[:, :len(occ[s])] * np.sqrt(np.asarray(occ[s]) / g)).astype(LD) for s in range(n_spin)]
+    Co = [np.asarray(C[s]).astype(LD)[:, :len(occ[s])] * np.sqrt(np.asarray(occ[s], dtype=LD) / LD(g))
+          for s in range(n_spin)]
     Ca = [Co[s][:, Co[s].shape[1] - n_active[s]:] for s in range(n_spin)]
     do_ind = indirect and k1 != 0.0
     out = {}
@@ -140,4 +143,6 @@ def energy_terms(ints, S, C, occ, n_active, k0, k1, indirect=True):
     out['e3_direct'], out['e3_indirect'] = e3d, e3i
     out['e4_direct'], out['e4_indirect'] = e4d, e4i
     out['total'] = e2d + e2i + e3d + e3i + e4d + e4i
+    if raw:
+        return out
     return {k: float(v) for k, v in out.items()}

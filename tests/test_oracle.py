@@ -243,3 +243,68 @@ def test_water_fixture_two_and_four_electron_terms(water):
     eb = brute.energy_terms(sys_['ints'], sys_['S'], sys_['orbitals'], sys_['occ'], sys_['n_active'], 1.0, 0.5)
     for k in ('e2_direct', 'e2_indirect', 'e4_direct', 'e4_indirect'):
         assert abs(eo[k] - eb[k]) <= 1e-9 * max(1e-3, abs(eb[k])), (k, eo[k], eb[k])
+
+
+# ---- frozen core: the reference's Fock is the derivative along paths that move only the active orbitals
+#
+# With n_active < n_occ the energy depends on the orbitals through two densities, D_all (every
+# occupied orbital: the k, l slots) and D_act (the active ones: the i, j slots).  The reference
+# returns F_all + F_act (four_electron.cpp:63-76,227-248 as 1/2 (fock + frozen_fock) of twice-weighted
+# slot sums; direct_utils.cpp:185-203; the active-index fock_i / fock_k sums, indirect_utils.cpp:311-342;
+# the active-only 2e term), which is dE/dt along any path that keeps the core fixed, where
+# dD_all = dD_act.  Along a path that also moves the core it is not: the second test shows the
+# restriction matters.
+
+FROZEN = [([4], [2]), ([4, 3], [3, 1]), ([5, 4], [3, 4])]
+
+
+def _fd_frozen(sys_, k0, k1, n_active, h=1e-4, n_dir=2, seed=3, core_too=False):
+    from helpers import active_only, density_derivative
+    rng = np.random.default_rng(seed)
+    _, F = oracle(sys_, k0, k1, n_active=n_active)
+    worst = 0.0
+    for _ in range(n_dir):
+        dC = [rng.standard_normal(c.shape) for c in sys_['orbitals']]
+        if not core_too:
+            dC = active_only(dC, sys_['occ'], n_active)
+
+        def E(t):
+            C = [c + t * d for c, d in zip(sys_['orbitals'], dC)]
+            return oracle(sys_, k0, k1, orbitals=C, n_active=n_active, fock=False)[0]['total']
+
+        fd = (8 * (E(h) - E(-h)) - (E(2 * h) - E(-2 * h))) / (12 * h)
+        an = density_derivative(F, sys_['orbitals'], sys_['occ'], dC)
+        worst = max(worst, abs(fd - an) / abs(fd))
+    return worst
+
+
+@pytest.mark.parametrize('n_occ,n_active', FROZEN)
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
+def test_oracle_frozen_core_fock_is_active_path_derivative(n_occ, n_active, kappa):
+    sys_ = synth.random_system(7, 10, 12, n_occ, n_active, seed=41 + sum(n_occ))
+    assert _fd_frozen(sys_, *kappa, n_active) < 1e-8
+
+
+def test_frozen_core_fock_is_not_the_core_path_derivative():
+    """Moving the core orbitals too, the same Fock misses the core-slot derivative: the
+    active-path check above is what pins it (a dropped or doubled slot fails it)."""
+    sys_ = synth.random_system(7, 10, 12, [4], [2], seed=45)
+    assert _fd_frozen(sys_, 1.0, 0.5, [2], core_too=True) > 1e-3
+
+
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
+def test_oracle_frozen_core_fock_recipe_long_double(kappa):
+    """c1 recipe with 4 of 5 orbitals active: active-only directions, long-double
+    Richardson differences of the energy (oracle/extended.py) against the oracle's FP64
+    Fock.  The bar is the oracle's own FP64 rounding on this recipe (explicit S+): measured
+    4.2e-9 .. 5.1e-9 relative to dE/dt; the GPU Fock is held to 1e-10 against the same
+    long-double differences (tests/test_gpu_fock_pins.py)."""
+    from helpers import active_only, density_derivative, ld_directional_derivative
+    sys_ = synth.make_system('c1')
+    rng = np.random.default_rng(7)
+    _, F = oracle(sys_, *kappa, n_active=[4])
+    for _ in range(2):
+        dC = active_only([rng.standard_normal(c.shape) for c in sys_['orbitals']], sys_['occ'], [4])
+        fd = ld_directional_derivative(sys_, *kappa, dC, n_active=[4])
+        an = density_derivative(F, sys_['orbitals'], sys_['occ'], dC)
+        assert abs(fd - an) <= 2e-8 * abs(fd), (fd, an)
