@@ -150,6 +150,28 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy);
 
+/* Size in doubles of the static device scratch array of uw12g_form_fock_noalloc
+ * (128 MiB per GPU the library runs on, held until the process ends). */
+#define UW12G_NOALLOC_SCRATCH_DOUBLES ((size_t)1 << 24)
+
+/* uw12g_form_fock without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, results and errors as uw12g_form_fock with stored integrals; every
+ * device buffer of the evaluation -- intermediates, split-K partial sums, and
+ * U^T Gamma of W / V when an earlier call has not kept it on the integrals --
+ * is carved from one static device array of UW12G_NOALLOC_SCRATCH_DOUBLES
+ * doubles per GPU, and the call makes no hipMalloc / hipFree.  An evaluation
+ * whose working set exceeds the array fails with -2 (the message names the
+ * size it needed) at the first buffer that does not fit, before that buffer is
+ * used.  All calls on one GPU share its array: calls that overlap on one GPU
+ * (from several threads or contexts) race.  Non-stored integrals: -2. */
+int uw12g_form_fock_noalloc(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                            const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C,
+                            size_t n_ao, size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ,
+                            const size_t* n_active, int indirect_term, double scale_opp_spin, double scale_same_spin,
+                            uw12g_precision precision, double* fock, double* terms_out, double* energy);
+
 /* Differentiation form, forward half: the energy and its six pieces as
  * uw12g_form_fock (fock == NULL), keeping the forward intermediates (the W/V
  * orbital transforms, (P|ij), t_AB, S+ Phi_V, X_AB) on the context.  *tape

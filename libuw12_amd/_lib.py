@@ -24,7 +24,7 @@ EXPORTS = (
     'uw12g_base_integrals_destroy',
     'uw12g_base_integrals_kept',
     'uw12g_setup_abs_projectors', 'uw12g_abs_projectors_destroy', 'uw12g_abs_projectors_get', 'uw12g_energy',
-    'uw12g_fock', 'uw12g_form_fock', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
+    'uw12g_fock', 'uw12g_form_fock', 'uw12g_form_fock_noalloc', 'uw12g_forward', 'uw12g_adjoint', 'uw12g_context_set_profiling',
     'uw12g_context_set_verbosity',
     'uw12g_context_profile',
     'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm', 'uw12g_test_set_stream_cap',
@@ -74,6 +74,7 @@ def load():
     L.uw12g_fock.argtypes = [vp, dp] + common + [ctypes.c_double, ctypes.c_double, dp]
     L.uw12g_form_fock.argtypes = [vp] + common + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp,
                                                   dp, dp]
+    L.uw12g_form_fock_noalloc.argtypes = L.uw12g_form_fock.argtypes
     L.uw12g_forward.argtypes = [vp] + common + [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, dp, dp,
                                                 ctypes.POINTER(ctypes.c_ulonglong)]
     L.uw12g_adjoint.argtypes = [vp, ctypes.c_ulonglong, dp]

@@ -42,6 +42,41 @@ static thread_local std::string g_last_error;
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 
+// Static device scratch of uw12g_form_fock_noalloc: each GPU the library runs
+// on holds its own copy until the process ends.  Host code never uses the
+// array's name as an address: scratch_address() returns the device address
+// hipGetSymbolAddress gave for the device, taken once per device when the
+// first context on it is created.
+static __device__ __align__(256) double scratch[UW12G_NOALLOC_SCRATCH_DOUBLES];
+
+namespace {
+std::mutex& scratch_mu() {
+  static std::mutex* mu = new std::mutex();
+  return *mu;
+}
+std::map<int, double*>& scratch_ptrs() {
+  static std::map<int, double*>* m = new std::map<int, double*>();
+  return *m;
+}
+}  // namespace
+
+// With `device` current: look up (once per device) the address of its scratch array.
+void init_scratch(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu());
+  if (scratch_ptrs().count(device)) return;
+  double* p = nullptr;
+  UW12G_HIP(hipGetSymbolAddress(reinterpret_cast<void**>(&p), HIP_SYMBOL(scratch)));
+  if (!p) throw Error(-3, "hipGetSymbolAddress returned no address for the scratch array");
+  register_alloc(p, sizeof(double) * UW12G_NOALLOC_SCRATCH_DOUBLES);
+  scratch_ptrs()[device] = p;
+}
+
+double* scratch_address(int device) {
+  std::lock_guard<std::mutex> lk(scratch_mu());
+  auto it = scratch_ptrs().find(device);
+  return it == scratch_ptrs().end() ? nullptr : it->second;
+}
+
 namespace {
 // Never destroyed: contexts held in static storage of a caller are destroyed
 // at process exit in an order this library does not control, and their buffers
@@ -83,6 +118,7 @@ void check_extent(const void* base, int64_t max_off, const char* what) {
 }
 
 double* Workspace::get(size_t n, hipStream_t stream) {
+  if (carve) return carve(owner, n);
   const size_t need = std::max<size_t>(n, 1) * sizeof(double);
   if (need > bytes) {
     if (ptr) {
@@ -163,7 +199,7 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
     for (const std::string& x : ctx->exact_extra)
       if (x == stage) exact = true;
   e.prec = ctx->prec == UW12G_MIXED ? (exact ? 0 : 1) : ctx->prec;
-  gemm_f64(e, ctx->stream, &ctx->gemm_ws, &ctx->prof);
+  gemm_f64(e, ctx->stream, ctx->ws_active, &ctx->prof);
 }
 
 // U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
@@ -171,12 +207,20 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
 // contraction over the ABS index runs over those r rows, S+ becoming the
 // diagonal 1/s (ri_utils.hpp:45-74 drops the other M - r directions).  Built
 // in FP64 once per (integrals, projector) pair and kept on the integrals.
-void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P) {
-  if (X->gt_id == P->id) return;
+// In the no-allocation mode a tensor not already kept is formed in the arena
+// for the one call (named `name`) and not kept.
+const double* rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P,
+                      const char* name) {
+  if (X->gt_id == P->id) return X->gt.p;
   const int M = P->M, N = X->N, Mr = P->r, D = X->D;
-  X->gt_id = 0;
-  if (X->gt.p && size_t(Mr) * N * D > X->gt.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
-  X->gt.get(size_t(std::max(Mr, 1)) * N * std::max(D, 1));
+  double* out = nullptr;
+  if (ctx->arena) {
+    out = ctx->buf(name, size_t(std::max(Mr, 1)) * N * std::max(D, 1));
+  } else {
+    X->gt_id = 0;
+    if (X->gt.p && size_t(Mr) * N * D > X->gt.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
+    out = X->gt.get(size_t(std::max(Mr, 1)) * N * std::max(D, 1));
+  }
   if (Mr > 0 && D > 0) {
     const int prec = ctx->prec;
     ctx->prec = 0;
@@ -192,14 +236,19 @@ void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw1
     g.Bk = lin(1);
     g.Bn = lin(M);
     g.Bb = int64_t(M) * N;
-    g.C = X->gt.p;
+    g.C = out;
     g.Cm = lin(1);
     g.Cn = lin(Mr);
     g.Cb = int64_t(Mr) * N;
     run_gemm(ctx, g, "abs_rotate");
     ctx->prec = prec;
   }
-  X->gt_id = P->id;
+  if (!ctx->arena) X->gt_id = P->id;
+  return out;
+}
+
+void ensure_rotated(uw12g_context* ctx, const uw12g_base_integrals* X, const uw12g_abs_projectors* P) {
+  (void)rotated(ctx, X, P, "gt");
 }
 
 }  // namespace uw12g
@@ -490,10 +539,16 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     require(!do_ind, "the same-spin terms need stored W and V integrals (indirect_term = 0 or scale_same_spin = 0)",
             -2);
     require(ctx->prec != UW12G_FP32, "the FP32 form needs stored W and V integrals", -2);
-  } else {
+  }
+  // U^T Gamma of W and V (kept on the integrals; in the no-allocation mode formed in the arena
+  // unless already kept)
+  const double* gtp[2] = {nullptr, nullptr};
+  if (!streamed) {
     require(WV->stored, "non-stored WV integrals need non-stored W and V integrals", -2);
-    ensure_rotated(ctx, W, P);
-    ensure_rotated(ctx, V, P);
+    gtp[0] = rotated(ctx, W, P, "gtW");
+    gtp[1] = rotated(ctx, V, P, "gtV");
+  } else {
+    require(!ctx->arena, "the no-allocation entry point needs stored integrals", -2);
   }
 
   // ---- host: occupation-weighted orbitals and active block (utils.hpp:235-310)
@@ -675,7 +730,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
     gp.N = q.o;
     gp.K = N;
     gp.batch = X->D;
-    gp.A = X->gt.p;
+    gp.A = gtp[op];
     gp.Am = lin(1);
     gp.Ak = lin(Mr);
     gp.Ab = int64_t(Mr) * N;
@@ -867,7 +922,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         a1.M = int(MrN);
         a1.N = na;
         a1.K = DV;
-        a1.A = V->gt.p;
+        a1.A = gtp[1];
         a1.Am = lin(1);
         a1.Ak = lin(int64_t(MrN));
         a1.B = Tb + A0;
@@ -883,7 +938,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         a3.K = Mr * na;
         a3.alpha = -1.0;
         a3.beta = 1.0;
-        a3.A = W->gt.p + MrN * A0;
+        a3.A = gtp[0] + MrN * A0;
         a3.Am = lin(Mr);
         a3.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
         a3.ks = P->dinv.p;
@@ -1379,7 +1434,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
             f.K = Mr * An;
             f.alpha = k1;
             f.beta = 1.0;
-            f.A = X->gt.p + MrN * A0;
+            f.A = gtp[pass] + MrN * A0;
             f.Am = lin(Mr);
             f.Ak = two(uint32_t(Mr), 1, int64_t(MrN));
             f.B = L;
@@ -1419,6 +1474,7 @@ int uw12g_context_create(int device, uw12g_context** out) {
     UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
     c->block_cap = std::max<size_t>(size_t(1) << 24, std::min<size_t>(total_b / 16, size_t(16) << 30) / 8);
     c->block_cap_default = c->block_cap;
+    init_scratch(device);
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Error(-3, "rocblas_create_handle failed");
     rocblas_set_stream(c->rb, c->stream);
     *out = c.release();
@@ -1615,6 +1671,54 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
     double* tp = terms_out ? terms_out : t6;
     PrecScope scope(ctx, int(precision));
     ++ctx->n_eval;
+    evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
+             scale_opp_spin, scale_same_spin, fock, tp, energy);
+  });
+}
+
+namespace {
+double* carve_workspace(void* owner, size_t n) { return static_cast<uw12g_context*>(owner)->carve("gemm_ws", n); }
+
+// The no-allocation mode for one call: named buffers and the GEMM workspace
+// come from the device's static scratch array; profiling (events) is off.
+struct ArenaScope {
+  uw12g_context* c;
+  bool prof_on;
+  ArenaScope(uw12g_context* ctx, double* base) : c(ctx), prof_on(ctx->prof.on) {
+    c->arena = base;
+    c->arena_cap = UW12G_NOALLOC_SCRATCH_DOUBLES;
+    c->arena_used = 0;
+    c->arena_bufs.clear();
+    c->arena_ws.carve = carve_workspace;
+    c->arena_ws.owner = c;
+    c->ws_active = &c->arena_ws;
+    c->prof.on = false;
+  }
+  ~ArenaScope() {
+    c->arena = nullptr;
+    c->arena_bufs.clear();
+    c->ws_active = &c->gemm_ws;
+    c->prof.on = prof_on;
+  }
+};
+}  // namespace
+
+int uw12g_form_fock_noalloc(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_base_integrals* V,
+                            const uw12g_base_integrals* WV, const uw12g_abs_projectors* abs, const double* C,
+                            size_t n_ao, size_t n_orb, size_t n_spin, const double* occ, const size_t* n_occ,
+                            const size_t* n_active, int indirect_term, double scale_opp_spin, double scale_same_spin,
+                            uw12g_precision precision, double* fock, double* terms_out, double* energy) {
+  return guard([&] {
+    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
+            -2);
+    require(ctx != nullptr, "null context");
+    double* base = scratch_address(ctx->device);
+    require(base != nullptr, "no scratch array on this device", -3);
+    double t6[6] = {0, 0, 0, 0, 0, 0};
+    double* tp = terms_out ? terms_out : t6;
+    PrecScope scope(ctx, int(precision));
+    ArenaScope arena(ctx, base);
+    // the context's named buffers (and so a forward call's tape) are not touched
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
              scale_opp_spin, scale_same_spin, fock, tp, energy);
   });

@@ -95,16 +95,43 @@ struct uw12g_context {
   // overwrites them in stream order, and keeping them allocated avoids a
   // multi-GiB allocate/free per evaluation.
   double* buf(const std::string& name, size_t n) {
+    if (arena) return carve(name, n);
     auto& b = bufs[name];
     if (!b) b.reset(new DBuf());
     if (b->p && n > b->n) UW12G_HIP(hipStreamSynchronize(stream));
     return b->get(n);
   }
   void release(const std::string& name) {
+    if (arena) return;
     auto it = bufs.find(name);
     if (it == bufs.end()) return;
     UW12G_HIP(hipStreamSynchronize(stream));
     bufs.erase(it);
+  }
+
+  // No-allocation mode (uw12g_form_fock_noalloc): while `arena` is set, named
+  // buffers and the GEMM workspace are carved, 256-byte aligned, from the
+  // device's static scratch array instead of allocated.  A name asked for again
+  // with at most its carved size gets the same region; a larger request gets a
+  // new one.  Exhausting the arena throws Error(-2) before the buffer is used.
+  double* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  std::map<std::string, std::pair<double*, size_t>> arena_bufs;
+  uw12g::Workspace arena_ws;
+  uw12g::Workspace* ws_active = &gemm_ws;
+  double* carve(const std::string& name, size_t n) {
+    n = std::max<size_t>(n, 1);
+    auto it = arena_bufs.find(name);
+    if (it != arena_bufs.end() && n <= it->second.second) return it->second.first;
+    const size_t off = (arena_used + 31) / 32 * 32;
+    if (off + n > arena_cap)
+      throw uw12g::Error(-2, "no-allocation scratch exhausted: this evaluation needs more than the " +
+                                 std::to_string(arena_cap) + " doubles of the static scratch array (buffer '" +
+                                 name + "' asks for " + std::to_string(n) + " more at offset " +
+                                 std::to_string(off) + ")");
+    arena_used = off + n;
+    arena_bufs[name] = {arena + off, n};
+    return arena + off;
   }
   ~uw12g_context() {
     prof.clear();

@@ -123,9 +123,13 @@ extern int g_force_tile[2];
 void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
 
 // Device scratch owned by a context: grown on demand, reused across calls.
+// With `carve` set the workspace never allocates: get() returns carve(n), a
+// region of memory the caller owns (the no-allocation entry point's arena).
 struct Workspace {
   void* ptr = nullptr;
   size_t bytes = 0;
+  double* (*carve)(void* owner, size_t n_doubles) = nullptr;
+  void* owner = nullptr;
   double* get(size_t n_doubles, hipStream_t stream);  // drains `stream` before growing
   ~Workspace();
 };
