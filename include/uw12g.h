@@ -187,9 +187,13 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
  * then beta, symmetric) of the forward call `tape`, from its kept
  * intermediates; the result equals uw12g_form_fock's Fock on the same inputs
  * (all-electron: the exact derivative; frozen core: the reference's averaged
- * form, four_electron.cpp:63-76).  Returns -4 when a later fused or forward
- * evaluation on the context has replaced the intermediates.  The integral and
- * projector handles of the forward call must still exist. */
+ * form, four_electron.cpp:63-76).  When the intermediates would exceed a
+ * quarter of the device memory, or the integrals are non-stored, the forward
+ * keeps none and the adjoint recomputes them (regenerating the three-index
+ * blocks for non-stored integrals); such a tape stays valid across later
+ * evaluations.  Otherwise returns -4 when a later fused or forward evaluation
+ * on the context has replaced the intermediates.  Destroying an integral or
+ * projector handle of the forward call invalidates its tape (-4). */
 int uw12g_adjoint(uw12g_context* ctx, unsigned long long tape, double* fock);
 
 /* Progress lines on stderr at the major stages of every evaluation on this

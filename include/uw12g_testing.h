@@ -50,6 +50,12 @@ int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles);
  * small values force many blocks (tests).  0 restores the default. */
 int uw12g_test_set_block_cap(uw12g_context* ctx, size_t doubles);
 
+/* Largest set of forward intermediates (doubles) a differentiation-form tape
+ * keeps on this context; past it uw12g_forward keeps nothing and
+ * uw12g_adjoint recomputes them (small values force that path; tests).  0
+ * restores the default, a quarter of the device memory. */
+int uw12g_test_set_tape_cap(uw12g_context* ctx, size_t doubles);
+
 /* Comma-separated stage names whose contractions the mixed form also runs in
  * FP64 on this context (error attribution); NULL or "" clears. */
 int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);

@@ -1474,6 +1474,7 @@ int uw12g_context_create(int device, uw12g_context** out) {
     UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
     c->block_cap = std::max<size_t>(size_t(1) << 24, std::min<size_t>(total_b / 16, size_t(16) << 30) / 8);
     c->block_cap_default = c->block_cap;
+    c->tape_cap = c->tape_cap_default = total_b / 4 / sizeof(double);
     init_scratch(device);
     if (rocblas_create_handle(&c->rb) != rocblas_status_success) throw Error(-3, "rocblas_create_handle failed");
     rocblas_set_stream(c->rb, c->stream);
@@ -1737,12 +1738,24 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
     require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
+    require(W && V && WV && abs, "null handle");
     PrecScope scope(ctx, int(precision));
     ++ctx->n_eval;
     ctx->tape.id = 0;  // no valid tape while the forward runs
+    // The intermediates the adjoint would read: per spin Phi (AO rows), Psi (rotated ABS rows) and
+    // (ik|A) of W and V, t and X, and diag(1/s) Psi_V.  Past the tape cap, or over non-stored
+    // integrals, the forward keeps nothing and the adjoint recomputes them.
+    const double DW = W->D, DV = V->D, Nao = double(n_ao), Mr = abs->r;
+    double kept = 0.0;
+    for (size_t s = 0; s < n_spin; ++s) {
+      const double o = double(n_occ[s]), a = double(n_active[s]);
+      kept += (Nao + Mr + a) * o * (DW + DV) + 2.0 * DW * DV + Mr * a * DV;
+    }
+    const bool recompute = !W->stored || !V->stored || kept > double(ctx->tape_cap);
     evaluate(ctx, W, V, WV, abs, C, n_ao, n_orb, n_spin, occ, n_occ, n_active, indirect_term != 0,
-             scale_opp_spin, scale_same_spin, nullptr, tp, energy, kForward);
+             scale_opp_spin, scale_same_spin, nullptr, tp, energy, recompute ? kFused : kForward);
     uw12g_context::Tape& t = ctx->tape;
+    t.recompute = recompute;
     size_t n_occ_tot = 0;
     for (size_t s = 0; s < n_spin; ++s) n_occ_tot += n_occ[s];
     t.W = W;
@@ -1771,9 +1784,17 @@ int uw12g_adjoint(uw12g_context* ctx, unsigned long long tape, double* fock) {
     require(ctx != nullptr && fock != nullptr, "null argument");
     const uw12g_context::Tape& t = ctx->tape;
     require(tape != 0 && tape == t.id, "unknown or superseded forward tape", -4);
-    require(t.n_eval == ctx->n_eval, "the forward intermediates were overwritten by a later evaluation", -4);
+    require(t.recompute || t.n_eval == ctx->n_eval,
+            "the forward intermediates were overwritten by a later evaluation", -4);
     PrecScope scope(ctx, t.prec);
     double t6[6];
+    if (t.recompute) {
+      // nothing kept: recompute every intermediate, i.e. the fused energy + Fock evaluation
+      ++ctx->n_eval;
+      evaluate(ctx, t.W, t.V, t.WV, t.P, t.C.data(), t.n_ao, t.n_orb, t.n_spin, t.occ.data(), t.n_occ.data(),
+               t.n_active.data(), t.indirect, t.k0, t.k1, fock, t6, nullptr, kFused);
+      return;
+    }
     evaluate(ctx, t.W, t.V, t.WV, t.P, t.C.data(), t.n_ao, t.n_orb, t.n_spin, t.occ.data(), t.n_occ.data(),
              t.n_active.data(), t.indirect, t.k0, t.k1, fock, t6, nullptr, kAdjoint);
   });
@@ -1807,6 +1828,13 @@ int uw12g_test_set_stream_cap(uw12g_context* ctx, size_t doubles) {
   return guard([&] {
     require(ctx != nullptr && doubles > 0, "invalid argument");
     ctx->stream_cap = doubles;
+  });
+}
+
+int uw12g_test_set_tape_cap(uw12g_context* ctx, size_t doubles) {
+  return guard([&] {
+    require(ctx != nullptr, "null context");
+    ctx->tape_cap = doubles > 0 ? doubles : ctx->tape_cap_default;
   });
 }
 

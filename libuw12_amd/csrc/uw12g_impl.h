@@ -75,7 +75,14 @@ struct uw12g_context {
     bool indirect = false;
     double k0 = 0.0, k1 = 0.0;
     int prec = 0;
+    // true: nothing is kept; the adjoint recomputes every intermediate (a fused
+    // evaluation), so the tape does not depend on the context's buffers
+    bool recompute = false;
   } tape;
+  // Largest set of forward intermediates (doubles) a tape keeps; past it the
+  // forward keeps nothing and the adjoint recomputes.  A quarter of the device
+  // memory by default.
+  size_t tape_cap = 0, tape_cap_default = 0;
   uint64_t n_eval = 0, n_tapes = 0;
   // Integral and projector handles still alive on this context.  Destroying the
   // context while some exist only marks it (doomed); the last handle's destroy

@@ -133,3 +133,53 @@ def test_adjoint_density_fd(gpu_ctx, n_occ):
             dD = d[:, :o] @ np.diag(occ) @ c[:, :o].T + c[:, :o] @ np.diag(occ) @ d[:, :o].T
             an += np.sum(Fs * dD)
         assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
+
+
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
+def test_recompute_tape_matches_kept_tape(gpu_ctx, kappa):
+    """With the tape cap forced below the intermediates' size the forward keeps nothing and
+    the adjoint recomputes them: the same energies and the same Fock as the kept tape, and
+    the tape stays valid across later evaluations."""
+    print(f'case recompute tape m6u kappa={kappa}', flush=True)
+    s = synth.make_system('m6u')
+    W, V, WV, P = gpu_setup(s, gpu_ctx)
+    args = (W, V, WV, P, s['orbitals'], s['occ'], s['n_active'])
+    kw = dict(indirect_term=kappa[1] != 0, scale_opp_spin=kappa[0], scale_same_spin=kappa[1])
+    e_kept, tape = u.forward(*args, **kw)
+    F_kept = u.adjoint(tape)
+    L = u._lib.load()
+    u._lib.check(L.uw12g_test_set_tape_cap(gpu_ctx._h, 1))
+    try:
+        e_rec, tape_r = u.forward(*args, **kw)
+        u.form_fock(*args, **kw)  # an evaluation in between leaves a recompute tape valid
+        F_rec = u.adjoint(tape_r)
+    finally:
+        u._lib.check(L.uw12g_test_set_tape_cap(gpu_ctx._h, 0))
+    for k in u.TERMS:
+        assert e_rec[k] == e_kept[k], k
+    for a, b in zip(F_rec, F_kept):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
+def test_streamed_forward_adjoint(gpu_ctx):
+    """Non-stored integrals: the forward keeps nothing; the adjoint regenerates the
+    three-index blocks (a streamed fused evaluation) and equals the stored adjoint."""
+    print('case streamed forward / adjoint m6u', flush=True)
+    s = synth.make_system('m6u')
+    Ws, Vs, WVs = synth.device_integrals(s, gpu_ctx, stored=False)
+    W, V, WV = synth.device_integrals(s, gpu_ctx, stored=True)
+    P = u.setup_abs_projectors(s['S'], s['N'], s['Nr'], context=gpu_ctx)
+    orb = (s['orbitals'], s['occ'], s['n_active'])
+    kw = dict(indirect_term=False, scale_opp_spin=1.0, scale_same_spin=0.0)
+    e_s, tape_s = u.forward(Ws, Vs, WVs, P, *orb, **kw)
+    F_s = u.adjoint(tape_s)
+    e_f, F_f = u.form_fock(Ws, Vs, WVs, P, *orb, **kw)
+    for k in u.TERMS:
+        assert e_s[k] == e_f[k], k
+    assert all(np.array_equal(a, b) for a, b in zip(F_s, F_f))
+    e_0, tape_0 = u.forward(W, V, WV, P, *orb, **kw)
+    F_0 = u.adjoint(tape_0)
+    for k in u.TERMS:
+        assert abs(e_s[k] - e_0[k]) <= 1e-11 * max(1.0, abs(e_0[k])), k
+    for a, b in zip(F_s, F_0):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(b).max()

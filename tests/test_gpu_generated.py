@@ -141,8 +141,8 @@ def test_streamed_rejects_unsupported(gpu_ctx):
     with pytest.raises(u.UW12Error) as ei:
         u.form_fock(Ws, Vs, WVs, P, *args, indirect_term=False, precision=u.FP32)
     assert ei.value.code == -2
-    with pytest.raises(u.UW12Error) as ei:
-        u.forward(Ws, Vs, WVs, P, *args, indirect_term=False)
+    with pytest.raises(u.UW12Error) as ei:  # the differentiation form shares the restriction
+        u.forward(Ws, Vs, WVs, P, *args, indirect_term=True, scale_opp_spin=1.0, scale_same_spin=0.5)
     assert ei.value.code == -2
     with pytest.raises(u.UW12Error) as ei:  # stored W / V with a non-stored WV
         u.form_fock(W, V, WVs, P, *args, indirect_term=False)
