@@ -10,7 +10,11 @@ intermediate tensors of the Fock terms, FP64 for the orbital transforms, the
 energies and every accumulation into the Fock matrix).  The plain FP64 path is
 timed beside it on the same inputs, and the mixed result is gated against it:
 `accuracy_gate` reports the largest energy-term and Fock deviations and
-whether both are within 1e-6.  `--form fp64` times the plain path instead.
+whether both are within 1e-6.  `--form fp64` times the plain path instead;
+`--form bf16x3` / `bf16x6` time the mixed form with its Fock intermediates as
+BF16 split products.  The side forms (FP32, mixed, BF16x3, BF16x6) are timed
+and compared with FP64 after the timed steps, and so is the differentiation
+form (forward then adjoint, `adjoint_form`).
 
 Workloads (--config): c1, c2 (host integral arrays, the default timed
 workload c2), c3 / c4 (integrals generated on the device and stored), c5
@@ -18,7 +22,10 @@ workload c2), c3 / c4 (integrals generated on the device and stored), c5
 evaluation regenerates its three-index blocks).  One c5 evaluation takes about
 half a minute, so the default run times c2 for the requested steps and then
 measures c5 at kappa = (1, 0) once per form (`c5_form`: mixed and FP64, the
-accuracy gate, its own roofline); `--c5-side skip` leaves that out.
+accuracy gate, its own roofline); `--c5-side skip` leaves that out.  Before
+that, `c3_forms` (`--c3-side skip` leaves it out) gives one evaluation per
+precision form at n_ao = 800 (unrestricted, kappa = (1, 0)): time and per-term
+and Fock deviation from FP64.
 
 Prints one JSON line (see the keys below).  --gpus > 1 is rejected: this
 build measures one GPU.
@@ -37,6 +44,7 @@ sys.path.insert(0, ROOT)
 METRIC = 'UW12 E+Fock evals/s, mixed vs FP64 on 1 MI355X at ≤1e-6 Eh; MFMA % peak'
 PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix peak (AMD published)
 PEAK_FP32_TFLOPS = 157.3  # MI355X dense FP32 matrix peak (AMD published)
+PEAK_BF16_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak (AMD published)
 PEAK_HBM_GBS = 8000.0
 GATE = 1e-6  # energy (Eh, per term and total) and Fock max-abs bar of the reduced-precision forms
 TERM_KEYS = ('e2_direct', 'e2_indirect', 'e3_direct', 'e3_indirect', 'e4_direct', 'e4_indirect', 'total')
@@ -97,9 +105,14 @@ def roofline_from_profile(records):
                           for k, v in sorted(stages.items(), key=lambda kv: -kv[1][0])}
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
-        # gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>: PREC 1 runs on the FP32 matrix cores
-        peak = PEAK_FP32_TFLOPS if dom.startswith('gemm_f64_kernel<') and dom.rstrip('>').endswith(',1') \
-            else PEAK_FP64_TFLOPS
+        # gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>: PREC 1 runs on the FP32 matrix cores.
+        # gemm_bf16x_kernel<..., planes>: 3 (two planes) or 6 (three planes) BF16 products per
+        # multiply-add, so the peak of the algorithmic FLOP rate is the BF16 peak over that count.
+        last = dom.rstrip('>').rsplit(',', 1)[-1]
+        if dom.startswith('gemm_bf16x_kernel<'):
+            peak = round(PEAK_BF16_TFLOPS / (3 if last == '2' else 6), 1)
+        else:
+            peak = PEAK_FP32_TFLOPS if dom.startswith('gemm_f64_kernel<') and last == '1' else PEAK_FP64_TFLOPS
         roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=peak, unit='TFLOP/s',
                     frac=round(achieved / peak, 4), traffic=None,
                     launches=n, avg_launch_us=round(1e3 * ms / n, 2),
@@ -247,7 +260,7 @@ def measure(name, system, handles, ctx, kappa, form, steps, warmup, side_forms, 
     occ_counts = [len(x) for x in system['occ']]
     indirect = abs(k1) > 4 * np.finfo(float).eps
     streamed = not getattr(W, 'stored', True)
-    prec = {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED}[form]
+    prec = PRECISION(u)[form]
 
     def evaluation(precision, fock=True):
         return u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
@@ -301,12 +314,36 @@ def measure(name, system, handles, ctx, kappa, form, steps, warmup, side_forms, 
               f'{dev["fock_max_abs_error"]:.3e}, gate {"met" if gate["met"] else "NOT met"}',
               file=sys.stderr, flush=True)
     others = {}
-    for fname, precision in (('fp32', u.FP32), ('mixed', u.MIXED)):
-        if fname == form or side_forms != 'auto' or (streamed and fname == 'fp32'):
+    for fname, precision in PRECISION(u).items():
+        if fname in (form, 'fp64') or side_forms != 'auto' or (streamed and fname == 'fp32'):
             continue
         tr, Fr, tp = timed_form(precision, fname)
-        others[f'{fname}_form'] = dict(deviation(tr, Fr, terms64, F64), evals_per_s=round(1.0 / tp, 6),
-                                       speedup_vs_fp64=round(t64 / tp, 3))
+        dev = deviation(tr, Fr, terms64, F64)
+        worst_e = max(abs(v) for v in dev['term_error_eh'].values())
+        others[f'{fname}_form'] = dict(dev, evals_per_s=round(1.0 / tp, 6), speedup_vs_fp64=round(t64 / tp, 3),
+                                       meets_gate=bool(worst_e <= GATE and dev['fock_max_abs_error'] <= GATE))
+    if side_forms == 'auto' and not streamed:
+        # differentiation form: forward (energies, intermediates kept or marked for
+        # recomputation) then the adjoint Fock, in the timed form and in FP64
+        def fwd_adj(precision, reps):
+            kw = dict(indirect_term=indirect, scale_opp_spin=k0, scale_same_spin=k1, precision=precision)
+            args = (W, V, WV, P, system['orbitals'], system['occ'], system['n_active'])
+            u.adjoint(u.forward(*args, **kw)[1])
+            t = time.perf_counter()
+            for _ in range(reps):
+                tr, tape = u.forward(*args, **kw)
+                Fa = u.adjoint(tape)
+            return tr, Fa, (time.perf_counter() - t) / reps
+        reps = max(1, side_reps)
+        ta64, Fa64, tp64 = fwd_adj(u.FP64, reps)
+        ta, Fa, tp = fwd_adj(prec, reps)
+        print(f'[bench] {label}: forward+adjoint {1e3 * tp:.1f} ms ({form}), {1e3 * tp64:.1f} ms (FP64)',
+              file=sys.stderr, flush=True)
+        dev = deviation(ta, Fa, terms64, F64)
+        others['adjoint_form'] = dict(
+            dev, form=form, evals_per_s=round(1.0 / tp, 6), ms_per_eval=round(1e3 * tp, 3),
+            fp64_ms_per_eval=round(1e3 * tp64, 3), speedup_vs_fp64=round(tp64 / tp, 3),
+            fp64_adjoint_vs_fused_fock_max_abs=max(float(np.abs(a - b).max()) for a, b in zip(Fa64, F64)))
     if energy_only and side_forms == 'auto':
         # the energy-only call (uw12_energy, libuw12.hpp:129-140) in the timed form
         reps = max(1, side_reps)
@@ -323,16 +360,57 @@ def measure(name, system, handles, ctx, kappa, form, steps, warmup, side_forms, 
                 occ_counts=occ_counts)
 
 
+def PRECISION(u):
+    return {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED, 'bf16x3': u.MIXED_BF16X3, 'bf16x6': u.MIXED_BF16X6}
+
+
+_GATE_TEXT = 'gated at <=1e-6 Eh per term and <=1e-6 Fock max-abs against the plain FP64 path of the same run'
 FORM_TEXT = {'fp64': 'plain FP64',
              'fp32': 'FP32 form (every contraction on FP32 matrix cores, FP64 storage and reductions)',
              'mixed': 'mixed form (FP32 matrix cores for the Fock intermediates K~, L, R, Xi, G, X_i/Y_i; FP64 '
                       'for the orbital transforms, the integral generation products, energies and Fock '
-                      'accumulation), gated at <=1e-6 Eh per term and <=1e-6 Fock max-abs against the plain FP64 '
-                      'path of the same run'}
+                      'accumulation), ' + _GATE_TEXT,
+             'bf16x3': 'mixed form with the Fock intermediates as BF16x3 split products (two bfloat16 planes per '
+                       'operand, three products, FP32 sums) on BF16 matrix cores, ' + _GATE_TEXT,
+             'bf16x6': 'mixed form with the Fock intermediates as BF16x6 split products (three bfloat16 planes per '
+                       'operand, six products, FP32 sums) on BF16 matrix cores, ' + _GATE_TEXT}
 INTEGRALS_TEXT = {'host': 'host integral arrays copied to HBM',
                   'device': 'integrals generated on the device, stored in HBM',
                   'streamed': 'integrals generated on the device, not stored: every evaluation regenerates its '
                               'three-index blocks (the rotated W tensor is kept between evaluations)'}
+
+
+def c3_forms(ctx):
+    """One evaluation of c3 (n_ao = 800, unrestricted, kappa = (1, 0), device-generated stored
+    integrals) per precision form after one untimed FP64 one: ms per evaluation and the
+    deviation of each reduced form from FP64."""
+    import libuw12_amd as u
+    kappa, integrals = CONFIG_DEFAULTS['c3']
+    system, (W, V, WV, P), t_setup = setup_config('c3', integrals, ctx)
+
+    def evaluation(precision):
+        t = time.perf_counter()
+        e, F = u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
+                           indirect_term=False, scale_opp_spin=kappa[0], scale_same_spin=kappa[1],
+                           precision=precision)
+        return e, F, time.perf_counter() - t
+
+    evaluation(u.FP64)
+    e64, F64, t64 = evaluation(u.FP64)
+    out = {'workload': workload('c3', system, kappa, 'fp64', integrals).split(', plain')[0],
+           'setup_s': round(t_setup, 2), 'fp64_ms_per_eval': round(1e3 * t64, 3)}
+    for fname, precision in PRECISION(u).items():
+        if fname == 'fp64':
+            continue
+        e, F, tp = evaluation(precision)
+        dev = deviation(e, F, e64, F64)
+        worst_e = max(abs(v) for v in dev['term_error_eh'].values())
+        out[fname] = dict(dev, ms_per_eval=round(1e3 * tp, 3), speedup_vs_fp64=round(t64 / tp, 3),
+                          meets_gate=bool(worst_e <= GATE and dev['fock_max_abs_error'] <= GATE))
+        print(f'[bench] c3: {fname} {1e3 * tp:.1f} ms, Fock max-abs error {dev["fock_max_abs_error"]:.2e}',
+              file=sys.stderr, flush=True)
+    del W, V, WV, P
+    return out
 
 
 def workload(name, system, kappa, form, integrals):
@@ -351,7 +429,7 @@ def main():
     ap.add_argument('--kappa', default=None, help='k0,k1 (default: (1,0.5) for c1/c2, (1,0) for c3-c5)')
     ap.add_argument('--integrals', default=None, choices=['host', 'device', 'streamed'],
                     help='default: host for c1/c2, device for c3/c4, streamed for c5')
-    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32'],
+    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32', 'bf16x3', 'bf16x6'],
                     help='precision form of the timed steps (the plain FP64 path is run beside it)')
     ap.add_argument('--side-reps', type=int, default=3,
                     help='timed evaluations of each side form (after one untimed one when more than 1)')
@@ -359,6 +437,8 @@ def main():
                     help='gate: only the FP64 evaluation the accuracy gate needs; skip: no FP64 / FP32 / mixed '
                          'evaluations beside the timed ones (no gate), for profiler runs that should see only the '
                          'timed form')
+    ap.add_argument('--c3-side', default='auto', choices=['auto', 'skip'],
+                    help='auto: after the timed c2 steps, one c3 evaluation per precision form')
     ap.add_argument('--c5-side', default='auto', choices=['auto', 'skip'],
                     help='auto: after the timed c2 steps, one c5 evaluation per form (mixed timed, FP64 gate)')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
@@ -423,7 +503,8 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': {'fp64': 'f64', 'fp32': 'f32', 'mixed': 'f32+f64'}[args.form],
+        'dtype': {'fp64': 'f64', 'fp32': 'f32', 'mixed': 'f32+f64', 'bf16x3': 'bf16x3+f64',
+                  'bf16x6': 'bf16x6+f64'}[args.form],
         'data': 'synthetic (seeded s-Gaussian cluster, analytic Boys-F0 integrals)',
         'config': {'workload': workload(args.config, system, kappa, args.form, integrals),
                    'form': args.form, 'kappa': list(kappa), 'integrals': integrals, 'parallelism': 'single-gpu'},
@@ -438,6 +519,12 @@ def main():
         'setup_s': round(t_setup, 2),
     }
     out.update(m['others'])
+    if args.config == 'c2' and args.c3_side == 'auto':
+        del handles, m
+        handles = m = None
+        ctx.close()
+        ctx = u.Context(0)
+        out['c3_forms'] = c3_forms(ctx)
     if args.config == 'c2' and args.c5_side == 'auto':
         # the n_ao = 1500 configuration the metric is quoted on: one timed evaluation per form
         del handles, m

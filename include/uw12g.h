@@ -39,8 +39,18 @@ typedef struct uw12g_abs_projectors uw12g_abs_projectors;
  * UW12G_MIXED runs the large intermediate tensors of the Fock terms that way
  * and keeps in FP64 the contractions whose results enter the energies directly
  * (orbital transforms, t, X, the energy-only products) and those that
- * accumulate into the Fock matrix. */
-typedef enum { UW12G_FP64 = 0, UW12G_FP32 = 1, UW12G_MIXED = 2 } uw12g_precision;
+ * accumulate into the Fock matrix.  UW12G_MIXED_BF16X3 and UW12G_MIXED_BF16X6
+ * are the mixed form with those intermediates as BF16 split products on the
+ * BF16 matrix cores (FP32 sums as above): each FP64 operand split into two
+ * bfloat16 planes and three plane products (about 3 * 2^-16 relative error
+ * per product), or into three planes and six products (about 2^-22). */
+typedef enum {
+  UW12G_FP64 = 0,
+  UW12G_FP32 = 1,
+  UW12G_MIXED = 2,
+  UW12G_MIXED_BF16X3 = 3,
+  UW12G_MIXED_BF16X6 = 4
+} uw12g_precision;
 
 /* Term order of terms_out[] */
 enum {
@@ -110,7 +120,7 @@ typedef int (*uw12g_three_index_gen_fn)(void* user, size_t df0, size_t n, int wi
  * direct_utils.cpp:349-436), so no three-index tensor of the size of X3_ri is
  * ever resident.  Evaluations with non-stored W / V integrals compute the
  * opposite-spin terms (indirect_term == 0 or scale_same_spin == 0; -2
- * otherwise) and take UW12G_FP64 or UW12G_MIXED. */
+ * otherwise) and take UW12G_FP64 or one of the mixed forms. */
 int uw12g_setup_base_integrals_gen(uw12g_context* ctx, const double* X2, uw12g_three_index_gen_fn gen, void* user,
                                    size_t n_ao, size_t n_df, size_t n_ri, int stored, uw12g_base_integrals** out);
 

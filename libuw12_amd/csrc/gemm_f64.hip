@@ -489,6 +489,397 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 split-product forms (prec 2 and 3).  Each FP64 operand is split into
+// bfloat16 planes -- prec 2: a ~ h + l, prec 3: a ~ h + m + l, each plane the
+// round-to-nearest bfloat16 of what the planes before it leave (Henry, Tang &
+// Heinecke, "Leveraging the bfloat16 artificial intelligence datatype for
+// higher-precision computations", ARITH 2019) -- and the product runs as the
+// plane products of order at most one (prec 2, three products: hh + hl + lh)
+// or two (prec 3, six products: + hm + mh + mm, with h, m, l in place of h, l)
+// on v_mfma_f32_16x16x32_bf16, accumulated in FP32 inside one launch chunk and
+// combined in FP64 across chunks like the FP32 form.  Per-product relative
+// error: about 3 * 2^-16 (prec 2), about 2^-22 (prec 3).
+//
+// K runs in 32-deep slabs (one MFMA k step).  Each thread owns octets of 8
+// consecutive k of one row (column): one 16-byte LDS write per plane and
+// octet, and one 16-byte LDS read fills an MFMA operand.  The LDS tile is
+// single-buffered: while the MFMAs of slab s run, the wave splits the FP64
+// values of slab s + 1 (loaded during slab s - 1) into packed plane words in
+// registers; after a barrier the words go to LDS and the loads of slab s + 2
+// are issued.
+constexpr int BKB = 32;
+constexpr int kRowWords = BKB / 2 + 4;  // plane row stride in 32-bit words: 80 bytes, 16-byte aligned
+
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef float fl2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+
+// Round-to-nearest bfloat16 of two FP32 values, packed (first in the low half).
+__device__ inline uint32_t pack_bf16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(fl2{a, b}, bf2));
+}
+__device__ inline float lo_f(uint32_t w) { return __builtin_bit_cast(float, w << 16); }
+__device__ inline float hi_f(uint32_t w) { return __builtin_bit_cast(float, w & 0xffff0000u); }
+
+// Plane words of two consecutive k values v0, v1.  Two planes: h from the FP32
+// value, l from the exact FP32 residual (the FP32 rounding of v, 2^-24, is far
+// below the form's 2^-16).  Three planes: the first residual is formed in FP64
+// (v - h is exact there) and rounded to FP32 (2^-24 of a residual of at most
+// 2^-8 |v|), the second residual exactly in FP32.
+template <int NP>
+__device__ inline void split_pair(double v0, double v1, uint32_t (&w)[NP]) {
+  const float f0 = float(v0), f1 = float(v1);
+  const uint32_t h = pack_bf16(f0, f1);
+  w[0] = h;
+  if constexpr (NP == 2) {
+    w[1] = pack_bf16(f0 - lo_f(h), f1 - hi_f(h));
+  } else {
+    const float r0 = float(v0 - double(lo_f(h))), r1 = float(v1 - double(hi_f(h)));
+    const uint32_t m = pack_bf16(r0, r1);
+    w[1] = m;
+    w[2] = pack_bf16(r0 - lo_f(m), r1 - hi_f(m));
+  }
+}
+
+template <int BM, int BN, bool AKC, bool BKC, bool KS, int NP>
+__global__ __launch_bounds__(NT) void gemm_bf16x_kernel(Params p) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int MB = WM / 16, NB = WN / 16;
+  constexpr int OA = BM / 64, OB = BN / 64;  // octets per thread and slab
+  constexpr int OT = OA + OB;
+  __shared__ __attribute__((aligned(16))) uint32_t As[NP][BM][kRowWords];
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[NP][BN][kRowWords];
+  const GemmDesc& d = p.d;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  int64_t w = blockIdx.x;
+  if (p.per > 0) {
+    w = int64_t(blockIdx.x & 7) * p.per + (blockIdx.x >> 3);
+    if (w >= p.work) return;
+  }
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int tile = int(w % tiles);
+  const int64_t q = w / tiles;
+  const int split = int(q % p.splits);
+  const int b = int(q / p.splits);
+  int tm, tn;
+  if (p.nfast) {
+    tm = tile / p.tiles_n;
+    tn = tile % p.tiles_n;
+  } else if (p.group > 1) {
+    const int per_group = p.group * p.tiles_n;
+    const int g = tile / per_group, r = tile - g * per_group;
+    const int first = g * p.group;
+    const int gs = min(p.tiles_m - first, p.group);
+    tm = first + r % gs;
+    tn = r / gs;
+  } else {
+    tm = tile % p.tiles_m;
+    tn = tile / p.tiles_m;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int k_begin = p.kbase + split * p.kchunk;
+  const int k_end = min(d.K, k_begin + p.kchunk);
+
+  const double* __restrict__ Ap = d.A + d.Ab * b;
+  const double* __restrict__ Bp = d.B + d.Bb * b;
+  const double* __restrict__ Kp = KS ? d.ks + d.Kb * b : nullptr;
+
+  // Octet assignment.  k along the lanes (AKC / BKC): octet t % 4 of the slab,
+  // rows t / 4 + 64 i.  Rows along the lanes: row t % BM, octets t / BM +
+  // (NT / BM) i (wave-uniform k).
+  auto a_row = [&](int i) { return AKC ? t / 4 + 64 * i : t % BM; };
+  auto b_col = [&](int i) { return BKC ? t / 4 + 64 * i : t % BN; };
+  auto a_oct = [&](int i) { return AKC ? t % 4 : t / BM + (NT / BM) * i; };
+  auto b_oct = [&](int i) { return BKC ? t % 4 : t / BN + (NT / BN) * i; };
+  // first k (slab-relative) of this thread's octets; wave-uniform with rows along the lanes
+  const int a_k0 = AKC ? 8 * (t % 4) : __builtin_amdgcn_readfirstlane(8 * (t / BM));
+  const int b_k0 = BKC ? 8 * (t % 4) : __builtin_amdgcn_readfirstlane(8 * (t / BN));
+  constexpr int DA = AKC ? 0 : 8 * (NT / BM), DB = BKC ? 0 : 8 * (NT / BN);  // k step between octets
+
+  double ra[OA][8], rb[OB][8], rs[AKC ? 1 : OA][8];
+  uint32_t pa[OA][NP][4], pb[OB][NP][4];
+
+  // split octet o (A octets first, then B) of the loaded slab into plane words
+  auto convert = [&](int o) {
+    if (o < OA) {
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        double v0 = ra[o][e], v1 = ra[o][e + 1];
+        if constexpr (KS) {
+          v0 *= rs[AKC ? 0 : o][e];
+          v1 *= rs[AKC ? 0 : o][e + 1];
+        }
+        uint32_t wd[NP];
+        split_pair<NP>(v0, v1, wd);
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) pa[o][pl][e / 2] = wd[pl];
+      }
+    } else {
+      const int ob = o - OA;
+#pragma unroll
+      for (int e = 0; e < 8; e += 2) {
+        uint32_t wd[NP];
+        split_pair<NP>(rb[ob][e], rb[ob][e + 1], wd);
+#pragma unroll
+        for (int pl = 0; pl < NP; ++pl) pb[ob][pl][e / 2] = wd[pl];
+      }
+    }
+  };
+  auto write_lds = [&]() {
+#pragma unroll
+    for (int i = 0; i < OA; ++i)
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        *reinterpret_cast<uint4*>(&As[pl][a_row(i)][4 * a_oct(i)]) =
+            uint4{pa[i][pl][0], pa[i][pl][1], pa[i][pl][2], pa[i][pl][3]};
+#pragma unroll
+    for (int i = 0; i < OB; ++i)
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        *reinterpret_cast<uint4*>(&Bs[pl][b_col(i)][4 * b_oct(i)]) =
+            uint4{pb[i][pl][0], pb[i][pl][1], pb[i][pl][2], pb[i][pl][3]};
+  };
+
+  f4 acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // The MFMAs of one slab from LDS, with (convc true) the conversion of the
+  // next slab's octets spread over the row blocks.  Lane (lr, lk) holds k =
+  // 8 lk .. 8 lk + 7 of row / column lr of each 16-wide block, the same k order
+  // in both operands.  Smallest plane products first.
+  constexpr int NPROD = NP == 3 ? 6 : 3;
+  auto compute = [&](auto convc) {
+    bf8 bfr[NB][NP];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        bfr[j][pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const uint4*>(&Bs[pl][wn0 + 16 * j + lr][4 * lk]));
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      bf8 afr[NP];
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        afr[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const uint4*>(&As[pl][wm0 + 16 * i + lr][4 * lk]));
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        f4 c = acc[i][j];
+        if constexpr (NP == 3) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][2], afr[0], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[2], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[1], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[1], c, 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[0], c, 0, 0, 0);
+      }
+      if constexpr (decltype(convc)::value) {
+#pragma unroll
+        for (int o = (i * OT) / MB; o < ((i + 1) * OT) / MB; ++o) convert(o);
+      }
+    }
+    if constexpr (decltype(convc)::value) {
+      // one slab as scheduling groups: the B operand reads, then per row block its
+      // A reads and its MFMAs, each MFMA followed by a share of the conversion
+      // (about 8 vector instructions per converted element pair and plane)
+      constexpr int kValuPerMfma = (OT * 8 * (NP == 3 ? 9 : 5) / 2 + MB * NB * NPROD - 1) / (MB * NB * NPROD);
+      __builtin_amdgcn_sched_group_barrier(0x100, NB * NP, 0);
+#pragma unroll
+      for (int i = 0; i < MB; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x100, NP, 0);
+#pragma unroll
+        for (int m = 0; m < NB * NPROD; ++m) {
+          __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+          __builtin_amdgcn_sched_group_barrier(0x002, kValuPerMfma, 0);
+        }
+      }
+    }
+  };
+
+  auto pipeline = [&](auto&& ld) {
+    ld(k_begin);
+#pragma unroll
+    for (int o = 0; o < OT; ++o) convert(o);
+    write_lds();
+    __syncthreads();
+    ld(k_begin + BKB);
+    const int nslab = (k_end - k_begin + BKB - 1) / BKB;
+    for (int s = 0; s + 1 < nslab; ++s) {
+      compute(std::true_type{});
+      __syncthreads();
+      write_lds();
+      ld(k_begin + (s + 2) * BKB);
+      __syncthreads();
+    }
+    compute(std::false_type{});
+  };
+
+  if (k_begin < k_end) {
+    if (p.fast) {
+      uint32_t a_lo[AKC ? OA : 1], b_lo[BKC ? OB : 1];
+#pragma unroll
+      for (int i = 0; i < (AKC ? OA : 1); ++i) {
+        const int m = m0 + a_row(i);
+        a_lo[i] = uint32_t(8 * ((m < d.M ? d.Am.off(m) : 0) + (AKC ? int64_t(a_k0) * d.Ak.s0 : 0)));
+      }
+#pragma unroll
+      for (int i = 0; i < (BKC ? OB : 1); ++i) {
+        const int n = n0 + b_col(i);
+        b_lo[i] = uint32_t(8 * ((n < d.N ? d.Bn.off(n) : 0) + (BKC ? int64_t(b_k0) * d.Bk.s0 : 0)));
+      }
+      const uint32_t k_lo = uint32_t(8 * (AKC ? int64_t(a_k0) * d.Km.s0 : 0));
+      const int k_last = k_end - BKB;
+      pipeline([&](int k0) {
+        if constexpr (AKC) {
+          const int kc = min(k0, k_last);  // look-ahead past the end re-reads the last slab
+          const auto r = rsrc(Ap + d.Ak.off(kc));
+          if constexpr (KS) {
+            const auto rk = rsrc(Kp + d.Km.off(kc));
+#pragma unroll
+            for (int e = 0; e < 8; ++e) rs[0][e] = ld_b(rk, k_lo, int32_t(8 * e * d.Km.s0));
+          }
+#pragma unroll
+          for (int i = 0; i < OA; ++i)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ra[i][e] = ld_b(r, a_lo[i], int32_t(8 * e * d.Ak.s0));
+        } else {
+          const int kb = k0 + a_k0;  // wave-uniform
+          const int64_t kbo = KS ? d.Km.off(kb) : 0;
+          const double* a0 = Ap + d.Ak.off(kb);
+          if (kb + DA * (OA - 1) + 7 < k_end) {  // every k of the slab in range (wave-uniform)
+            const auto r0 = rsrc(a0);
+#pragma unroll
+            for (int i = 0; i < OA; ++i)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) {
+                const int dk = DA * i + e;
+                ra[i][e] = ld_b(r0, a_lo[0], int32_t(8 * dk * d.Ak.s0));
+                if constexpr (KS) rs[i][e] = Kp[kbo + dk * d.Km.s0];
+              }
+          } else {
+#pragma unroll
+            for (int i = 0; i < OA; ++i)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) {
+                const int dk = DA * i + e;
+                const bool kok = kb + dk < k_end;
+                ra[i][e] = ld_b(rsrc(a0, kok), a_lo[0], int32_t(8 * dk * d.Ak.s0));
+                if constexpr (KS) rs[i][e] = Kp[kok ? kbo + dk * d.Km.s0 : 0];
+              }
+          }
+        }
+        if constexpr (BKC) {
+          const auto r = rsrc(Bp + d.Bk.off(min(k0, k_last)));
+#pragma unroll
+          for (int i = 0; i < OB; ++i)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) rb[i][e] = ld_b(r, b_lo[i], int32_t(8 * e * d.Bk.s0));
+        } else {
+          const int kb = k0 + b_k0;
+          const double* b0 = Bp + d.Bk.off(kb);
+          if (kb + DB * (OB - 1) + 7 < k_end) {
+            const auto r0 = rsrc(b0);
+#pragma unroll
+            for (int i = 0; i < OB; ++i)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) rb[i][e] = ld_b(r0, b_lo[0], int32_t(8 * (DB * i + e) * d.Bk.s0));
+          } else {
+#pragma unroll
+            for (int i = 0; i < OB; ++i)
+#pragma unroll
+              for (int e = 0; e < 8; ++e) {
+                const int dk = DB * i + e;
+                const bool kok = kb + dk < k_end;
+                rb[i][e] = ld_b(rsrc(b0, kok), b_lo[0], int32_t(8 * dk * d.Bk.s0));
+              }
+          }
+        }
+      });
+    } else {
+      // general path: 64-bit offsets, clamped in-bounds addresses and zeroed
+      // values for k past the end
+      int64_t a_moff[AKC ? OA : 1], b_noff[BKC ? OB : 1];
+#pragma unroll
+      for (int i = 0; i < (AKC ? OA : 1); ++i) {
+        const int m = m0 + a_row(i);
+        a_moff[i] = m < d.M ? d.Am.off(m) : 0;
+      }
+#pragma unroll
+      for (int i = 0; i < (BKC ? OB : 1); ++i) {
+        const int n = n0 + b_col(i);
+        b_noff[i] = n < d.N ? d.Bn.off(n) : 0;
+      }
+      pipeline([&](int k0) {
+#pragma unroll
+        for (int i = 0; i < OA; ++i)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int k = k0 + a_k0 + DA * i + e;
+            const bool kok = k < k_end;
+            const double v = Ap[a_moff[AKC ? i : 0] + (kok ? d.Ak.off(k) : 0)];
+            ra[i][e] = kok ? v : 0.0;
+            if constexpr (KS) {
+              if (!AKC || i == 0) rs[AKC ? 0 : i][e] = Kp[kok ? d.Km.off(k) : 0];
+            }
+          }
+#pragma unroll
+        for (int i = 0; i < OB; ++i)
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int k = k0 + b_k0 + DB * i + e;
+            const bool kok = k < k_end;
+            const double v = Bp[b_noff[BKC ? i : 0] + (kok ? d.Bk.off(k) : 0)];
+            rb[i][e] = kok ? v : 0.0;
+          }
+      });
+    }
+  }
+
+  // epilogue: the FP32 16x16 output layout, C[m = .. + lr][n = .. + 4 lk + r]
+  auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + 4 * lk + r; };
+  if (p.out) {
+    double* W = p.out + (int64_t(split) * d.batch + b) * int64_t(d.M) * d.N;
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      const int m = m0 + wm0 + 16 * i + lr;
+      if (m >= d.M) continue;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = ncol(j, r);
+          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * double(acc[i][j][r]);
+        }
+    }
+    return;
+  }
+  double* Cp = d.C + d.Cb * b;
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    const int m = m0 + wm0 + 16 * i + lr;
+    if (m >= d.M) continue;
+    const int64_t mo = d.Cm.off(m);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = ncol(j, r);
+        if (n >= d.N) continue;
+        double* c = Cp + mo + d.Cn.off(n);
+        const double v = d.alpha * double(acc[i][j][r]);
+        *c = d.beta == 0.0 ? v : v + d.beta * *c;
+      }
+  }
+}
+
 // Sum split-K partials and apply beta / the output map (fixed order: deterministic).
 __global__ void splitk_reduce_kernel(GemmDesc d, const double* W, int splits) {
   const int64_t MN = int64_t(d.M) * d.N;
@@ -521,10 +912,13 @@ __global__ void gemm_naive_kernel(GemmDesc d) {
 
 // Profiling name, matching the template arguments of the instantiation
 // (gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>).
+// (gemm_bf16x_kernel<BM, BN, AKC, BKC, KS, planes> for the BF16 split forms).
 template <int BM, int BN>
 std::string cfg_name(bool akc, bool bkc, bool ks, int prec) {
-  return "gemm_f64_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
-         (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + "," + std::to_string(prec) + ">";
+  const std::string tail = std::to_string(BM) + "," + std::to_string(BN) + "," + (akc ? "1" : "0") + "," +
+                           (bkc ? "1" : "0") + "," + (ks ? "1" : "0") + ",";
+  if (prec >= 2) return "gemm_bf16x_kernel<" + tail + std::to_string(prec) + ">";
+  return "gemm_f64_kernel<" + tail + std::to_string(prec) + ">";
 }
 
 template <int BM, int BN, bool KS, int PREC>
@@ -535,15 +929,35 @@ void launch_ks(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
   else hipLaunchKernelGGL((gemm_f64_kernel<BM, BN, false, false, KS, PREC>), grid, dim3(NT), 0, s, p);
 }
 
+template <int BM, int BN, bool KS, int NP>
+void launch_bf(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
+  if (akc && bkc) hipLaunchKernelGGL((gemm_bf16x_kernel<BM, BN, true, true, KS, NP>), grid, dim3(NT), 0, s, p);
+  else if (akc) hipLaunchKernelGGL((gemm_bf16x_kernel<BM, BN, true, false, KS, NP>), grid, dim3(NT), 0, s, p);
+  else if (bkc) hipLaunchKernelGGL((gemm_bf16x_kernel<BM, BN, false, true, KS, NP>), grid, dim3(NT), 0, s, p);
+  else hipLaunchKernelGGL((gemm_bf16x_kernel<BM, BN, false, false, KS, NP>), grid, dim3(NT), 0, s, p);
+}
+
 template <int BM, int BN>
 void launch_cfg(const Params& p, bool akc, bool bkc, dim3 grid, hipStream_t s) {
-  if (p.d.prec == 1) {
+  if (p.d.prec >= 2) {
+    const bool ks = p.d.ks != nullptr;
+    if (p.d.prec == 2) ks ? launch_bf<BM, BN, true, 2>(p, akc, bkc, grid, s) : launch_bf<BM, BN, false, 2>(p, akc, bkc, grid, s);
+    else ks ? launch_bf<BM, BN, true, 3>(p, akc, bkc, grid, s) : launch_bf<BM, BN, false, 3>(p, akc, bkc, grid, s);
+  } else if (p.d.prec == 1) {
     if (p.d.ks) launch_ks<BM, BN, true, 1>(p, akc, bkc, grid, s);
     else launch_ks<BM, BN, false, 1>(p, akc, bkc, grid, s);
   } else {
     if (p.d.ks) launch_ks<BM, BN, true, 0>(p, akc, bkc, grid, s);
     else launch_ks<BM, BN, false, 0>(p, akc, bkc, grid, s);
   }
+}
+
+template <int BM, int BN, bool KS, int NP>
+const void* kernel_ptr_bf(bool akc, bool bkc) {
+  if (akc && bkc) return reinterpret_cast<const void*>(&gemm_bf16x_kernel<BM, BN, true, true, KS, NP>);
+  if (akc) return reinterpret_cast<const void*>(&gemm_bf16x_kernel<BM, BN, true, false, KS, NP>);
+  if (bkc) return reinterpret_cast<const void*>(&gemm_bf16x_kernel<BM, BN, false, true, KS, NP>);
+  return reinterpret_cast<const void*>(&gemm_bf16x_kernel<BM, BN, false, false, KS, NP>);
 }
 
 template <int BM, int BN, bool KS, int PREC>
@@ -563,8 +977,10 @@ int64_t gemm_slots(bool akc, bool bkc, bool ks, int prec) {
   std::lock_guard<std::mutex> lock(mu);
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
-  const void* f = prec ? (ks ? kernel_ptr_ks<BM, BN, true, 1>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 1>(akc, bkc))
-                       : (ks ? kernel_ptr_ks<BM, BN, true, 0>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 0>(akc, bkc));
+  const void* f = prec == 3   ? (ks ? kernel_ptr_bf<BM, BN, true, 3>(akc, bkc) : kernel_ptr_bf<BM, BN, false, 3>(akc, bkc))
+                  : prec == 2 ? (ks ? kernel_ptr_bf<BM, BN, true, 2>(akc, bkc) : kernel_ptr_bf<BM, BN, false, 2>(akc, bkc))
+                  : prec      ? (ks ? kernel_ptr_ks<BM, BN, true, 1>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 1>(akc, bkc))
+                              : (ks ? kernel_ptr_ks<BM, BN, true, 0>(akc, bkc) : kernel_ptr_ks<BM, BN, false, 0>(akc, bkc));
   int per_cu = 0, dev = 0, cus = 0;
   UW12G_HIP(hipGetDevice(&dev));
   UW12G_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -608,9 +1024,12 @@ int choose_splits(int64_t wgs, int kt, int K, int batch, int64_t slots) {
 // Unit stride on the low level of a map (contiguous runs along that index).
 bool unit(const Map2& m) { return m.s0 == 1 && m.n0 >= 2; }
 
+// k depth of one slab: 16 for the FP64 and FP32 forms, 32 for the BF16 split forms.
+int slab_depth(int prec) { return prec >= 2 ? BKB : BK; }
+
 // A k map is affine inside every slab when its inner period is a multiple of
-// BK (slabs start at multiples of BK) or it has a single level.
-bool slab_affine(const Map2& m) { return m.n0 == 0xffffffffu || m.n0 % BK == 0; }
+// the slab depth bk (slabs start at multiples of it) or it has a single level.
+bool slab_affine(const Map2& m, int bk) { return m.n0 == 0xffffffffu || m.n0 % bk == 0; }
 bool nonneg(const Map2& m) { return m.s0 >= 0 && m.s1 >= 0; }
 }  // namespace
 int64_t map_max(const Map2& m, int64_t extent);
@@ -624,15 +1043,17 @@ namespace {
 constexpr int64_t kLaneLim = (int64_t(1) << 28) - 1;
 
 bool fast_maps_ok(const GemmDesc& d) {
-  if (!slab_affine(d.Ak) || !slab_affine(d.Bk) || (d.ks && !slab_affine(d.Km))) return false;
+  const int bk = slab_depth(d.prec);
+  if (!slab_affine(d.Ak, bk) || !slab_affine(d.Bk, bk) || (d.ks && !slab_affine(d.Km, bk))) return false;
   return nonneg(d.Am) && nonneg(d.Ak) && nonneg(d.Bk) && nonneg(d.Bn) && (!d.ks || nonneg(d.Km));
 }
 
 bool fast_ok(const GemmDesc& d, bool akc, bool /*bkc*/) {
   if (!fast_maps_ok(d)) return false;
-  const int64_t a = map_max(d.Am, d.M) + (BK - 1) * d.Ak.s0;
-  const int64_t b = map_max(d.Bn, d.N) + (BK - 1) * d.Bk.s0;
-  const int64_t k = (d.ks && akc) ? (BK - 1) * d.Km.s0 : 0;
+  const int bk = slab_depth(d.prec);
+  const int64_t a = map_max(d.Am, d.M) + (bk - 1) * d.Ak.s0;
+  const int64_t b = map_max(d.Bn, d.N) + (bk - 1) * d.Bk.s0;
+  const int64_t k = (d.ks && akc) ? (bk - 1) * d.Km.s0 : 0;
   return a < kLaneLim && b < kLaneLim && k < kLaneLim;
 }
 
@@ -725,10 +1146,11 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   const int klen = d.K - kbase;
   int splits = 1;
   const int64_t wgs = tiles * d.batch;
-  const int kt = (klen + BK - 1) / BK;
+  const int bk = slab_depth(d.prec);
+  const int kt = (klen + bk - 1) / bk;
   if (kbase == 0 && kt >= 16 && ws)
     splits = choose_splits(wgs, kt, klen, d.batch, slots_for(BM, BN, akc, bkc, d.ks != nullptr, d.prec));
-  if (d.prec == 1 && klen > kF32Chunk) {
+  if (d.prec >= 1 && klen > kF32Chunk) {
     // FP32 accumulation runs over at most kF32Chunk k values; chunks add up in FP64
     const int nchunk = (klen + kF32Chunk - 1) / kF32Chunk;
     if (kbase != 0 || !ws || int64_t(std::max(splits, nchunk)) * d.batch * d.M * d.N > kSplitWsMax) {
@@ -746,7 +1168,7 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
     splits = std::max(splits, nchunk);
     if (splits > 65535) throw Error(-2, "gemm: too many FP32 k chunks");
   }
-  p.kchunk = ((kt + splits - 1) / splits) * BK;
+  p.kchunk = ((kt + splits - 1) / splits) * bk;
   splits = (klen + p.kchunk - 1) / p.kchunk;
   p.splits = splits;
   p.kbase = kbase;
@@ -809,9 +1231,10 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   // split the output into blocks of rows / columns, each launched from a 64-bit
   // base pointer (A, B and C shifted to the block's first row / column), so the
   // lane offsets of every block stay small and the block keeps the fast path.
-  if (d.K >= BK && fast_maps_ok(d) && !fast_ok(d, akc, bkc)) {
-    const int64_t ka = (BK - 1) * d.Ak.s0, kb = (BK - 1) * d.Bk.s0;
-    const bool kfit = !(d.ks && akc) || (BK - 1) * d.Km.s0 < kLaneLim;
+  const int bk = slab_depth(d.prec);
+  if (d.K >= bk && fast_maps_ok(d) && !fast_ok(d, akc, bkc)) {
+    const int64_t ka = (bk - 1) * d.Ak.s0, kb = (bk - 1) * d.Bk.s0;
+    const bool kfit = !(d.ks && akc) || (bk - 1) * d.Km.s0 < kLaneLim;
     const int64_t cm = chunk_len(d.Am, d.Cm, d.M, kLaneLim - 1 - ka);
     const int64_t cn = chunk_len(d.Bn, d.Cn, d.N, kLaneLim - 1 - kb);
     if (kfit && cm > 0 && cn > 0 && (cm < d.M || cn < d.N)) {
@@ -830,8 +1253,8 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   }
   // with k along the lanes the unpredicated path needs whole slabs: the k
   // remainder then runs on the general path, accumulated onto them
-  const int krem = (akc || bkc) ? d.K % BK : 0;
-  if (d.K >= BK && fast_ok(d, akc, bkc)) {
+  const int krem = (akc || bkc) ? d.K % bk : 0;
+  if (d.K >= bk && fast_ok(d, akc, bkc)) {
     GemmDesc m = d;
     m.K = d.K - krem;
     launch_gemm(m, 0, true, akc, bkc, stream, ws, prof);

@@ -81,7 +81,7 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
     if (naive == 1) {
       gemm_f64_naive(d, nullptr);
     } else {
-      d.prec = naive == 2 ? 1 : 0;
+      d.prec = naive >= 2 ? naive - 1 : 0;  // 2: FP32, 3: BF16x3, 4: BF16x6
       gemm_f64(d, nullptr, &ws);
     }
     UW12G_HIP(hipDeviceSynchronize());
@@ -102,7 +102,7 @@ extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int prec, int reps, double* ms) {
   try {
-    if (prec != 0 && prec != 1) throw Error(-1, "prec must be 0 (FP64) or 1 (FP32)");
+    if (prec < 0 || prec > 3) throw Error(-1, "prec must be 0 (FP64), 1 (FP32), 2 (BF16x3) or 3 (BF16x6)");
     if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw Error(-1, "M, N, K and batch must be positive");
     // every element the descriptor addresses lies inside the caller's arrays (the
     // maps are checked non-negative, so the largest offset is the sum of the maxima)

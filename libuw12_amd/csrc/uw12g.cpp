@@ -195,10 +195,13 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   ctx->prof.stage = stage;
   GemmDesc e = d;
-  if (ctx->prec == UW12G_MIXED && !exact)
+  const bool mixed = ctx->prec >= UW12G_MIXED;
+  if (mixed && !exact)
     for (const std::string& x : ctx->exact_extra)
       if (x == stage) exact = true;
-  e.prec = ctx->prec == UW12G_MIXED ? (exact ? 0 : 1) : ctx->prec;
+  // GEMM arithmetic of the reduced contractions: 1 FP32, 2 / 3 BF16 with two / three planes
+  const int reduced = ctx->prec == UW12G_MIXED_BF16X3 ? 2 : ctx->prec == UW12G_MIXED_BF16X6 ? 3 : 1;
+  e.prec = mixed ? (exact ? 0 : reduced) : ctx->prec;
   gemm_f64(e, ctx->stream, ctx->ws_active, &ctx->prof);
 }
 
@@ -1665,8 +1668,7 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
-            -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
     require(ctx != nullptr, "null context");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
@@ -1710,8 +1712,7 @@ int uw12g_form_fock_noalloc(uw12g_context* ctx, const uw12g_base_integrals* W, c
                             const size_t* n_active, int indirect_term, double scale_opp_spin, double scale_same_spin,
                             uw12g_precision precision, double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
-            -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
     require(ctx != nullptr, "null context");
     double* base = scratch_address(ctx->device);
     require(base != nullptr, "no scratch array on this device", -3);
@@ -1731,8 +1732,7 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
                   int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                   double* terms_out, double* energy, unsigned long long* tape) {
   return guard([&] {
-    require(precision == UW12G_FP64 || precision == UW12G_FP32 || precision == UW12G_MIXED, "unsupported precision",
-            -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
     require(ctx != nullptr && tape != nullptr, "null argument");
     require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
     require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");

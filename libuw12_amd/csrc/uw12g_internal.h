@@ -67,7 +67,7 @@ inline Map2 two(uint32_t n0, int64_t s0, int64_t s1) {
 
 // C(m,n) <- alpha sum_k A(m,k) ks(k) B(k,n) + beta C(m,n), for every batch b.
 // A(m,k) = A[Ab*b + Am.off(m) + Ak.off(k)], and likewise for B, C, ks.
-// Longest k range a reduced-precision (prec 1) contraction accumulates in FP32;
+// Longest k range a reduced-precision (prec >= 1) contraction accumulates in FP32;
 // longer sums are split into such ranges and the partial results added in FP64.
 constexpr int kF32Chunk = 4096;
 
@@ -86,7 +86,9 @@ struct GemmDesc {
   const double* ks = nullptr;
   Map2 Km;
   int64_t Kb = 0;
-  int prec = 0;  // matrix-core arithmetic: 0 FP64, 1 FP32 (operands and chunk sums in FP32, FP64 storage)
+  // matrix-core arithmetic: 0 FP64, 1 FP32 (operands and chunk sums in FP32, FP64 storage),
+  // 2 / 3 BF16 split products with two / three planes per operand (FP32 chunk sums)
+  int prec = 0;
 };
 
 // Allocation registry: every device buffer the library allocates is recorded

@@ -7,7 +7,11 @@ of alpha), kappa = (1, 0), integrals generated on the device and stored.
   headline metric);
 * FP32 form: each term within 1e-6 of its size (e2 direct is a ~10^2 Eh term)
   and the Fock within 1e-4 -- its errors are reported (measured ~2e-5 Eh on
-  e2 / e3 direct, ~1e-5 on the Fock)."""
+  e2 / e3 direct, ~1e-5 on the Fock);
+* BF16 split forms (the mixed form with its Fock intermediates as BF16x6 /
+  BF16x3 products): energies within 1e-6 Eh of FP64 (their paths stay FP64
+  as in the mixed form); BF16x6 Fock within the 1e-6 gate,
+  BF16x3 Fock within 1e-4, errors reported."""
 import numpy as np
 import pytest
 
@@ -49,3 +53,11 @@ def test_c3_precision_forms(c3):
           flush=True)
     assert 0.0 < dm <= 1e-6
     assert d32 <= 1e-4
+    for form, name, bar in ((u.MIXED_BF16X6, 'BF16x6', 1e-6), (u.MIXED_BF16X3, 'BF16x3', 1e-4)):
+        print(f'case c3 {name}', flush=True)
+        eb, Fb = u.form_fock(*args, **kw, precision=form)
+        for k in TERMS + ('total',):
+            assert abs(eb[k] - e64[k]) <= 1e-6, (name, k, eb[k], e64[k])
+        db = fock_error(Fb, F64)
+        print(f'  Fock max-abs: {name} {db:.2e}', flush=True)
+        assert 0.0 < db <= bar

@@ -1,4 +1,4 @@
-"""The strided MFMA GEMM kernel (FP64 and FP32 forms) against the plain kernel and numpy."""
+"""The strided MFMA GEMM kernel (FP64, FP32 and BF16 split-product forms) against the plain kernel and numpy."""
 import numpy as np
 import pytest
 
@@ -112,6 +112,33 @@ def fp32_bound(M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, 
     return g * absC
 
 
+# Per-product relative error of the BF16 split forms (test hook modes 3 and 4) and
+# the FP32 terms summed per k.  Mode 3 (two planes, hh + hl + lh): each operand
+# is h + l to 2^-16 (h = bf16(v), l = bf16(v - h), unit roundoff 2^-8 per plane)
+# and the dropped ll is below 2^-16: 3 * 2^-16 in all.  Mode 4 (three planes, six
+# products hh hm mh hl mm lh): operands to about 2^-24, dropped ml + lm + ll below
+# 2^-23: 2^-22 in all.  Small factors cover the FP32 roundings inside the splits.
+BF16_FORMS = {3: (3.1 * 2.0 ** -16, 3), 4: (1.1 * 2.0 ** -22, 6)}
+
+
+def reduced_bound(mode, M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, ks=None, Km=None,
+                  chunk=4096):
+    """Elementwise bound on |C - C_exact| for a reduced form: mode 2 the FP32 form
+    (fp32_bound), modes 3 / 4 the BF16 split forms: (per-product error + gamma_n)
+    times |A| |B|, with n the FP32 additions of one k chunk (products per k times
+    the chunk length)."""
+    if mode == 2:
+        return fp32_bound(M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, ks, Km, chunk)
+    rel, per_k = BF16_FORMS[mode]
+    u = 2.0 ** -24
+    n = per_k * min(K, chunk) + 2
+    g = n * u / (1 - n * u)
+    absC = host_gemm(M, N, K, batch, abs(alpha), 0.0, np.abs(A), Am, Ak, Ab, np.abs(B), Bk, Bn, Bb,
+                     np.zeros(_c_size(M, N, batch, Cm, Cn, Cb)), Cm, Cn, Cb,
+                     None if ks is None else np.abs(ks), Km)
+    return (rel + g) * absC
+
+
 def _c_size(M, N, batch, Cm, Cn, Cb):
     def mx(m, e):
         if e <= 0:
@@ -146,16 +173,45 @@ def test_fp32_form_within_rounding_bound(name, M, N, K, batch, layout):
     assert np.abs(got - ref).max() > 0
 
 
-@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('name,M,N,K,batch', CASES)
+@pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
+@pytest.mark.parametrize('mode', [3, 4])
+def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode):
+    """BF16 split forms (two and three planes) within their error bound; the
+    three-plane form is within a small factor of the FP32 form's error."""
+    rng = np.random.default_rng(hash(('bf16', name, layout, mode)) % 2**32)
+    A = rng.standard_normal(M * K * batch)
+    B = rng.standard_normal(K * N * batch)
+    C = rng.standard_normal(M * N * batch)
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    args = (M, N, K, batch, 1.25, 0.5, A, Am, Ak, M * K, B, Bk, Bn, K * N, C, M2(1), M2(M), M * N)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, Am, Ak, M * K, B, Bk, Bn, K * N, M2(1), M2(M), M * N)
+    err = np.abs(got - ref)
+    bound = bound + 1e-15 * np.abs(ref)  # the beta C term, in FP64
+    assert np.all(err <= bound), (err / bound).max()
+    assert err.max() > 0
+    if mode == 4 and K >= 16:
+        e32 = np.abs(run(*args, naive=2) - ref).max()
+        assert err.max() <= 8 * e32, (err.max(), e32)
+
+
+@pytest.mark.parametrize('mode', [0, 2, 3, 4])
 @pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
 @pytest.mark.parametrize('kc', [False, True])
-def test_affine_two_level_k_maps(mode, K, kc):
-    """Compound k = (r, q) with an inner period of 32 (a multiple of the 16-deep
-    slab): the unpredicated buffer-load path, in both the k-along-waves form
-    (kc False) and the k-along-lanes form (kc True, where a k remainder runs as
-    a second launch), with a k scale."""
-    rng = np.random.default_rng(17 + K + 2 * kc + mode)
-    M, N, n0 = 200, 150, 32
+@pytest.mark.parametrize('n0', [32, 16])
+def test_affine_two_level_k_maps(mode, K, kc, n0):
+    """Compound k = (r, q) with an inner period n0: with n0 a multiple of the slab
+    depth (16 for FP64 / FP32, 32 for the BF16 forms) the unpredicated buffer-load
+    path, in both the k-along-waves form (kc False) and the k-along-lanes form (kc
+    True, where a k remainder runs as a second launch), with a k scale; with n0 = 16
+    the BF16 forms take the general path."""
+    if n0 == 16 and mode < 3:
+        pytest.skip('n0 = 16 is already slab-affine for the 16-deep slabs')
+    rng = np.random.default_rng(17 + K + 2 * kc + mode + n0)
+    M, N = 200, 150
     q = (K + n0 - 1) // n0
     # A(m, k) = A[m*sm + (k % n0)*s0 + (k // n0)*s1]
     if kc:
@@ -174,12 +230,13 @@ def test_affine_two_level_k_maps(mode, K, kc):
     if mode == 0:
         assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
     else:
-        bound = fp32_bound(M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, Km)
-        assert np.all(np.abs(got - ref) <= bound)
+        bound = reduced_bound(mode, M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, Km)
+        # the beta C term is exact
+        assert np.all(np.abs(got - ref) <= bound + 1e-15 * np.abs(ref))
 
 
 @pytest.mark.parametrize('side', ['A', 'B'])
-@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('mode', [0, 2, 3, 4])
 def test_operand_span_past_lane_offsets(side, mode):
     """An operand whose row (A) or column (B) offsets pass 2^28 doubles, the limit of the
     buffer path's 32-bit lane offsets: the launcher splits the output into row / column
@@ -205,5 +262,9 @@ def test_operand_span_past_lane_offsets(side, mode):
     args = (M, N, K, 1, 0.75, 0.5, A, Am, Ak, 0, B, Bk, Bn, 0, C, Cm, Cn, 0)
     ref = host_gemm(*args)
     got = run(*args, naive=mode)
-    tol = 1e-12 if mode == 0 else 1e-5
-    assert np.abs(got - ref).max() <= tol * np.abs(ref).max()
+    if mode == 0:
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    else:
+        bound = reduced_bound(mode, M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, Cm, Cn, 0)
+        bound = np.pad(bound, (0, C.size - bound.size))  # elements past the output map are untouched
+        assert np.all(np.abs(got - ref) <= bound + 1e-15 * np.abs(ref))

@@ -247,6 +247,37 @@ def test_mixed_form_errors(gpu_ctx, name, kappa):
     assert 0.0 < fock_error(Fm, F64) <= 1e-6
 
 
+@pytest.mark.parametrize('name', ['c1', 'm6u', 'm12'])
+@pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
+def test_bf16_split_forms_errors(gpu_ctx, name, kappa):
+    """Mixed form with the Fock intermediates as BF16 split products, against FP64.
+    As in the mixed form, every energy term but the same-spin 3e one is
+    bit-identical.  Three planes (per-product error ~2^-22, below FP32's
+    rounding of the operands): the accuracy gate, 1e-6 Eh per term and 1e-6
+    Fock max-abs.  Two planes (~3 * 2^-16 per product): e3 indirect within 1e-5
+    Eh and the Fock within 5e-5 (measured 7e-7 and 5e-6 at n_ao = 320), and no
+    more accurate than the three-plane form."""
+    print(f'case {name} kappa={kappa} BF16 split forms', flush=True)
+    s = synth.make_system(name)
+    W, V, WV, P = gpu_setup(s, gpu_ctx)
+    args = (W, V, WV, P, s['orbitals'], s['occ'], s['n_active'])
+    kw = dict(indirect_term=True, scale_opp_spin=kappa[0], scale_same_spin=kappa[1])
+    e64, F64 = u.form_fock(*args, **kw, precision=u.FP64)
+    errs = {}
+    for form, e_bar, f_bar in ((u.MIXED_BF16X6, 1e-6, 1e-6), (u.MIXED_BF16X3, 1e-5, 5e-5)):
+        e, F = u.form_fock(*args, **kw, precision=form)
+        for k in TERMS:
+            if k == 'e3_indirect' and kappa[1] != 0.0:
+                assert abs(e[k] - e64[k]) <= e_bar, (form, k, e[k], e64[k])
+            else:
+                assert e[k] == e64[k], (form, k, e[k], e64[k])
+        errs[form] = fock_error(F, F64)
+        print(f'  form {form}: e3 indirect {e["e3_indirect"] - e64["e3_indirect"]:.2e}  Fock {errs[form]:.2e}',
+              flush=True)
+        assert 0.0 < errs[form] <= f_bar
+    assert errs[u.MIXED_BF16X6] <= errs[u.MIXED_BF16X3]
+
+
 def test_fp32_form_errors(gpu_ctx):
     """FP32 form against FP64 on the c1 recipe: every contraction rounds its
     operands to FP32 and sums in FP32 over k chunks of at most 4096, so each
