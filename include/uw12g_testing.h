@@ -65,6 +65,13 @@ int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);
  * tile-choice measurements). */
 int uw12g_test_force_tile(int bm, int bn);
 
+/* BF16 split forms: on != 0 (the default) splits the operands of each product
+ * once into bfloat16 plane tensors in the workspace; 0 splits them inside the
+ * GEMM kernel, for every later launch in the process (tests of both paths).
+ * max_bytes > 0 caps the plane workspace of one product, so that larger
+ * products run in pieces; 0 restores the default cap (16 GiB). */
+int uw12g_test_set_bf16_planes(int on, long long max_bytes);
+
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);
 

@@ -25,6 +25,8 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 int g_force_tile[2] = {0, 0};
+int g_bf16_planes = 1;
+int64_t g_plane_bytes_max = 0;
 
 namespace {
 
@@ -512,6 +514,7 @@ constexpr int BKB = 32;
 constexpr int kRowWords = BKB / 2 + 4;  // plane row stride in 32-bit words: 80 bytes, 16-byte aligned
 
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float fl2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
@@ -880,6 +883,254 @@ __global__ __launch_bounds__(NT) void gemm_bf16x_kernel(Params p) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// BF16 plane tensors.  For the split forms the operands are split once per
+// product into packed bfloat16 planes (split_planes_kernel), so that the GEMM
+// (gemm_bf16p_kernel) reads 2 bytes per element and plane instead of the 8 of
+// the FP64 value, with no conversion in its main loop.  Layout of an operand
+// viewed as X(r, k) (r: the output row for A, the output column for B) over
+// nb batches: 16-byte units holding k = 8 q .. 8 q + 7 of one r in one plane,
+// unit ((plane nb + b) KQ + q) Rp + r, with Rp the row count rounded up to
+// 128 and KQ = 4 ceil(K / 32), padding zero.  A 32-deep slab of a 128-row
+// tile is then 4 runs of 128 consecutive units (2 KiB) per plane.
+struct PlaneDesc {
+  const double* X = nullptr;
+  Map2 Xr, Xk;
+  int64_t Xb = 0;
+  const double* ks = nullptr;  // optional k scale (A operand)
+  Map2 Km;
+  int64_t Kb = 0;
+  int R = 0, K = 0, nb = 1, KQ = 0, Rp = 0;
+  u32x4* out = nullptr;
+};
+
+// One thread per unit, r fastest (the unit writes of a wave are contiguous).
+template <int NP>
+__global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s) {
+  const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
+  const int64_t plane = units;
+  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < units; u += int64_t(gridDim.x) * blockDim.x) {
+    const int r = int(u % s.Rp);
+    const int64_t qb = u / s.Rp;
+    const int q = int(qb % s.KQ), b = int(qb / s.KQ);
+    const bool rok = r < s.R;
+    const int64_t base = s.Xb * b + (rok ? s.Xr.off(r) : 0);
+    double v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * q + e;
+      const bool ok = rok && k < s.K;
+      double x = s.X[ok ? base + s.Xk.off(k) : 0];
+      if (s.ks) x *= s.ks[ok ? s.Kb * b + s.Km.off(k) : 0];
+      v[e] = ok ? x : 0.0;
+    }
+    uint32_t w[4][NP];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) s.out[pl * plane + u] = u32x4{w[0][pl], w[1][pl], w[2][pl], w[3][pl]};
+  }
+}
+
+struct PlaneArgs {
+  const u32x4* A = nullptr;
+  const u32x4* B = nullptr;
+  int nbA = 1, nbB = 1, RpA = 0, RpB = 0, KQ = 0;
+};
+
+// GEMM over plane tensors: the tile order, split-K partials and epilogue of
+// gemm_bf16x_kernel; k slabs of 32 loaded as 16-byte units (4 BM / 256 per
+// thread and plane) into registers two slabs ahead, written to the same LDS
+// layout, the same MFMA plane products.
+template <int BM, int BN, int NP>
+__global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) {
+  constexpr int WM = BM / 2, WN = BN / 2;
+  constexpr int MB = WM / 16, NB = WN / 16;
+  constexpr int UA = 4 * BM / NT, UB = 4 * BN / NT;  // units per thread, plane and slab
+  __shared__ __attribute__((aligned(16))) uint32_t As[NP][BM][kRowWords];
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[NP][BN][kRowWords];
+  const GemmDesc& d = p.d;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  int64_t w = blockIdx.x;
+  if (p.per > 0) {
+    w = int64_t(blockIdx.x & 7) * p.per + (blockIdx.x >> 3);
+    if (w >= p.work) return;
+  }
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int tile = int(w % tiles);
+  const int64_t q = w / tiles;
+  const int split = int(q % p.splits);
+  const int b = int(q / p.splits);
+  int tm, tn;
+  if (p.nfast) {
+    tm = tile / p.tiles_n;
+    tn = tile % p.tiles_n;
+  } else if (p.group > 1) {
+    const int per_group = p.group * p.tiles_n;
+    const int g = tile / per_group, r = tile - g * per_group;
+    const int first = g * p.group;
+    const int gs = min(p.tiles_m - first, p.group);
+    tm = first + r % gs;
+    tn = r / gs;
+  } else {
+    tm = tile % p.tiles_m;
+    tn = tile / p.tiles_m;
+  }
+  const int m0 = tm * BM, n0 = tn * BN;
+  const int k_begin = p.kbase + split * p.kchunk;
+  const int k_end = min(d.K, k_begin + p.kchunk);
+
+  const int64_t planeA = int64_t(pa.nbA) * pa.KQ * pa.RpA, planeB = int64_t(pa.nbB) * pa.KQ * pa.RpB;
+  const u32x4* __restrict__ Ab = pa.A + int64_t(pa.nbA > 1 ? b : 0) * pa.KQ * pa.RpA + m0;
+  const u32x4* __restrict__ Bb = pa.B + int64_t(pa.nbB > 1 ? b : 0) * pa.KQ * pa.RpB + n0;
+  const int q0 = k_begin / 8;  // k_begin is a multiple of 32
+  const int nslab = (k_end - k_begin + BKB - 1) / BKB;
+
+  u32x4 xa[NP][UA], xb[NP][UB], ya[NP][UA], yb[NP][UB];  // ya, yb unused with three planes
+  // unit i of this thread: row (t + NT i) % BM, slab-relative q (t + NT i) / BM
+  auto ld = [&](u32x4(&ra)[NP][UA], u32x4(&rb)[NP][UB], int s) {
+    const int qs = q0 + 4 * min(s, nslab - 1);
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+#pragma unroll
+      for (int i = 0; i < UA; ++i) {
+        const int u = t + NT * i;
+        ra[pl][i] = Ab[pl * planeA + int64_t(qs + u / BM) * pa.RpA + u % BM];
+      }
+#pragma unroll
+      for (int i = 0; i < UB; ++i) {
+        const int u = t + NT * i;
+        rb[pl][i] = Bb[pl * planeB + int64_t(qs + u / BN) * pa.RpB + u % BN];
+      }
+    }
+  };
+  auto wr = [&](const u32x4(&ra)[NP][UA], const u32x4(&rb)[NP][UB]) {
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) {
+#pragma unroll
+      for (int i = 0; i < UA; ++i) {
+        const int u = t + NT * i;
+        *reinterpret_cast<u32x4*>(&As[pl][u % BM][4 * (u / BM)]) = ra[pl][i];
+      }
+#pragma unroll
+      for (int i = 0; i < UB; ++i) {
+        const int u = t + NT * i;
+        *reinterpret_cast<u32x4*>(&Bs[pl][u % BN][4 * (u / BN)]) = rb[pl][i];
+      }
+    }
+  };
+
+  f4 acc[MB][NB];
+#pragma unroll
+  for (int i = 0; i < MB; ++i)
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[i][j] = f4{0, 0, 0, 0};
+  const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
+  const int lr = lane & 15, lk = lane >> 4;
+  auto compute = [&]() {
+    bf8 bfr[NB][NP];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        bfr[j][pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const u32x4*>(&Bs[pl][wn0 + 16 * j + lr][4 * lk]));
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      bf8 afr[NP];
+#pragma unroll
+      for (int pl = 0; pl < NP; ++pl)
+        afr[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const u32x4*>(&As[pl][wm0 + 16 * i + lr][4 * lk]));
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        f4 c = acc[i][j];
+        if constexpr (NP == 3) {
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][2], afr[0], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[2], c, 0, 0, 0);
+          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[1], c, 0, 0, 0);
+        }
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[0], c, 0, 0, 0);
+        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[1], c, 0, 0, 0);
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[0], c, 0, 0, 0);
+      }
+    }
+  };
+
+  if (nslab > 0 && NP == 3) {
+    // three planes: one register set (two would leave one wave per SIMD);
+    // LDS holds slab s while the loads of slab s + 1 are in flight
+    ld(xa, xb, 0);
+    wr(xa, xb);
+    __syncthreads();
+    ld(xa, xb, 1);
+    for (int s = 0;; ++s) {
+      compute();
+      if (s + 1 >= nslab) break;
+      __syncthreads();
+      wr(xa, xb);
+      __syncthreads();
+      ld(xa, xb, s + 2);
+    }
+  } else if (nslab > 0) {
+    // LDS holds slab s while the registers of one set hold slab s + 1 and the
+    // other the loads of slab s + 2 in flight
+    ld(xa, xb, 0);
+    ld(ya, yb, 1);
+    wr(xa, xb);
+    __syncthreads();
+    ld(xa, xb, 2);
+    for (int s = 0;; s += 2) {
+      compute();
+      if (s + 1 >= nslab) break;
+      __syncthreads();
+      wr(ya, yb);
+      __syncthreads();
+      ld(ya, yb, s + 3);
+      compute();
+      if (s + 2 >= nslab) break;
+      __syncthreads();
+      wr(xa, xb);
+      __syncthreads();
+      ld(xa, xb, s + 4);
+    }
+  }
+
+  auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + 4 * lk + r; };
+  if (p.out) {
+    double* W = p.out + (int64_t(split) * d.batch + b) * int64_t(d.M) * d.N;
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+      const int m = m0 + wm0 + 16 * i + lr;
+      if (m >= d.M) continue;
+#pragma unroll
+      for (int j = 0; j < NB; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = ncol(j, r);
+          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * double(acc[i][j][r]);
+        }
+    }
+    return;
+  }
+  double* Cp = d.C + d.Cb * b;
+#pragma unroll
+  for (int i = 0; i < MB; ++i) {
+    const int m = m0 + wm0 + 16 * i + lr;
+    if (m >= d.M) continue;
+    const int64_t mo = d.Cm.off(m);
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int n = ncol(j, r);
+        if (n >= d.N) continue;
+        double* c = Cp + mo + d.Cn.off(n);
+        const double v = d.alpha * double(acc[i][j][r]);
+        *c = d.beta == 0.0 ? v : v + d.beta * *c;
+      }
+  }
+}
+
 // Sum split-K partials and apply beta / the output map (fixed order: deterministic).
 __global__ void splitk_reduce_kernel(GemmDesc d, const double* W, int splits) {
   const int64_t MN = int64_t(d.M) * d.N;
@@ -1212,6 +1463,240 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   }
 }
 
+
+// ---- BF16 split forms over plane tensors
+
+// Largest plane workspace (bytes) one product may use (g_plane_bytes_max);
+// larger products run in pieces over the batch, the output rows or the output
+// columns.
+constexpr int64_t kPlaneBytesMax = int64_t(16) << 30;
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+template <int NP>
+void launch_split(const PlaneDesc& s, hipStream_t stream, Profiler* prof) {
+  const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
+  const unsigned grid = unsigned(std::min<int64_t>((units + 255) / 256, 8192));
+  const bool timing = prof && prof->on;
+  ProfileRec rec;
+  if (timing) prof->begin(rec, stream);
+  hipLaunchKernelGGL(split_planes_kernel<NP>, dim3(grid), dim3(256), 0, stream, s);
+  UW12G_HIP(hipGetLastError());
+  if (timing)
+    prof->end(rec, stream, "split_planes_kernel<" + std::to_string(NP) + ">", 0.0,
+              (8.0 + 2.0 * NP) * double(s.R) * s.K * s.nb);
+}
+
+template <int BM, int BN>
+void launch_bf16p_cfg(const Params& p, const PlaneArgs& pa, int np, dim3 grid, hipStream_t s) {
+  if (np == 2) hipLaunchKernelGGL((gemm_bf16p_kernel<BM, BN, 2>), grid, dim3(NT), 0, s, p, pa);
+  else hipLaunchKernelGGL((gemm_bf16p_kernel<BM, BN, 3>), grid, dim3(NT), 0, s, p, pa);
+}
+
+template <int BM, int BN>
+int64_t bf16p_slots(int np) {
+  static std::mutex mu;
+  static std::map<int, int64_t> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = cache.find(np);
+  if (it != cache.end()) return it->second;
+  const void* f = np == 2 ? reinterpret_cast<const void*>(&gemm_bf16p_kernel<BM, BN, 2>)
+                          : reinterpret_cast<const void*>(&gemm_bf16p_kernel<BM, BN, 3>);
+  int per_cu = 0, dev = 0, cus = 0;
+  UW12G_HIP(hipGetDevice(&dev));
+  UW12G_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  UW12G_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, 0));
+  const int64_t slots = int64_t(std::max(per_cu, 1)) * std::max(cus, 1);
+  cache[np] = slots;
+  return slots;
+}
+
+// One product C = alpha A B + beta C of a split form over plane tensors, or
+// false when the workspace cannot hold its planes (then the caller runs the
+// in-kernel split of gemm_bf16x_kernel).  Large products run in pieces.
+bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
+  if (!ws || ws->carve) return false;
+  const int np = d.prec == 2 ? 2 : 3;
+  const int nbA = (d.Ab == 0 && (!d.ks || d.Kb == 0)) ? 1 : d.batch;
+  const int nbB = d.Bb == 0 ? 1 : d.batch;
+  const int64_t KQ = 4 * ((int64_t(d.K) + BKB - 1) / BKB);
+  const int64_t RpA = round_up(d.M, 128), RpB = round_up(d.N, 128);
+  const int64_t bytesA = int64_t(np) * nbA * KQ * RpA * 16, bytesB = int64_t(np) * nbB * KQ * RpB * 16;
+  if (KQ > 0x7fffffff / 8) return false;
+
+  // tile shape as for the other forms
+  const bool small = int64_t((d.M + 63) / 64) * ((d.N + 63) / 64) * d.batch <= 64;
+  int BM = 128, BN = 128;
+  double best = -1.0;
+  for (int bm : {128, 64})
+    for (int bn : {128, 64}) {
+      if (g_force_tile[0] && (bm != g_force_tile[0] || bn != g_force_tile[1])) continue;
+      const double padded = double((d.M + bm - 1) / bm * bm) * double((d.N + bn - 1) / bn * bn);
+      const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? (small ? 0.9 : 0.7)
+                                                                                    : (small ? 0.95 : 0.9);
+      const double score = speed * double(d.M) * double(d.N) / padded;
+      if (score > best + 1e-12) {
+        best = score;
+        BM = bm;
+        BN = bn;
+      }
+    }
+  Params p;
+  p.d = d;
+  p.tiles_m = (d.M + BM - 1) / BM;
+  p.tiles_n = (d.N + BN - 1) / BN;
+  const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
+  if (tiles > 0x7fffffff) return false;
+  const int kt = int(KQ / 4);
+  const int64_t slots = BM == 128 ? (BN == 128 ? bf16p_slots<128, 128>(np) : bf16p_slots<128, 64>(np))
+                                  : (BN == 128 ? bf16p_slots<64, 128>(np) : bf16p_slots<64, 64>(np));
+  int splits = kt >= 16 ? choose_splits(tiles * d.batch, kt, d.K, d.batch, slots) : 1;
+  // FP32 sums over at most kF32Chunk k values: that many chunks at least
+  const int nchunk = (d.K + kF32Chunk - 1) / kF32Chunk;
+  bool sequential = false;
+  if (nchunk > 1) {
+    if (int64_t(std::max(splits, nchunk)) * d.batch * d.M * d.N > kSplitWsMax) sequential = true;
+    else splits = std::max(splits, nchunk);
+  }
+  if (sequential) splits = 1;
+  const int64_t part = splits > 1 ? int64_t(splits) * d.batch * d.M * d.N * 8 : 0;
+  const int64_t need = round_up(bytesA, 256) + round_up(bytesB, 256) + part;
+
+  // workspace budget: what it holds already, or half of the free device memory
+  size_t free_b = 0, total_b = 0;
+  UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
+  const int64_t cap = g_plane_bytes_max > 0 ? g_plane_bytes_max : kPlaneBytesMax;
+  const int64_t budget =
+      std::min<int64_t>(cap, std::max<int64_t>(int64_t(ws->bytes), int64_t(ws->bytes) + int64_t(free_b) / 2));
+  if (need > budget) {
+    // pieces: batch halves, else row / column halves (whole periods of the maps)
+    if (d.batch > 1) {
+      const int h = d.batch / 2;
+      for (int b0 : {0, h}) {
+        GemmDesc e = d;
+        e.batch = b0 == 0 ? h : d.batch - h;
+        e.A = d.A + d.Ab * b0;
+        e.B = d.B + d.Bb * b0;
+        e.C = d.C + d.Cb * b0;
+        if (d.ks) e.ks = d.ks + d.Kb * b0;
+        if (!gemm_bf16_planes(e, stream, ws, prof)) gemm_f64(e, stream, nullptr, prof);
+      }
+      return true;
+    }
+    const bool rows = bytesA >= bytesB;
+    const Map2& om = rows ? d.Am : d.Bn;
+    const Map2& cm = rows ? d.Cm : d.Cn;
+    const int64_t ext = rows ? d.M : d.N;
+    const int64_t pa_ = period(om, ext), pc_ = period(cm, ext);
+    const int64_t per = pa_ / std::gcd(pa_, pc_) * pc_;
+    int64_t half = round_up((ext + 1) / 2, std::max<int64_t>(per, 128));
+    if (half >= ext || ext <= 128) return false;
+    for (int64_t x0 = 0; x0 < ext; x0 += half) {
+      GemmDesc e = d;
+      const int len = int(std::min<int64_t>(half, ext - x0));
+      if (rows) {
+        e.M = len;
+        e.A = d.A + d.Am.off(uint32_t(x0));
+        e.C = d.C + d.Cm.off(uint32_t(x0));
+      } else {
+        e.N = len;
+        e.B = d.B + d.Bn.off(uint32_t(x0));
+        e.C = d.C + d.Cn.off(uint32_t(x0));
+      }
+      if (!gemm_bf16_planes(e, stream, ws, prof)) gemm_f64(e, stream, nullptr, prof);
+    }
+    return true;
+  }
+
+  char* base = reinterpret_cast<char*>(ws->get(size_t((need + 7) / 8), stream));
+  PlaneArgs pa;
+  pa.A = reinterpret_cast<const u32x4*>(base);
+  pa.B = reinterpret_cast<const u32x4*>(base + round_up(bytesA, 256));
+  double* partials = part ? reinterpret_cast<double*>(base + round_up(bytesA, 256) + round_up(bytesB, 256)) : nullptr;
+  pa.nbA = nbA;
+  pa.nbB = nbB;
+  pa.RpA = int(RpA);
+  pa.RpB = int(RpB);
+  pa.KQ = int(KQ);
+  PlaneDesc sa;
+  sa.X = d.A;
+  sa.Xr = d.Am;
+  sa.Xk = d.Ak;
+  sa.Xb = d.Ab;
+  sa.ks = d.ks;
+  sa.Km = d.Km;
+  sa.Kb = d.Kb;
+  sa.R = d.M;
+  sa.K = d.K;
+  sa.nb = nbA;
+  sa.KQ = int(KQ);
+  sa.Rp = int(RpA);
+  sa.out = const_cast<u32x4*>(pa.A);
+  PlaneDesc sb;
+  sb.X = d.B;
+  sb.Xr = d.Bn;
+  sb.Xk = d.Bk;
+  sb.Xb = d.Bb;
+  sb.R = d.N;
+  sb.K = d.K;
+  sb.nb = nbB;
+  sb.KQ = int(KQ);
+  sb.Rp = int(RpB);
+  sb.out = const_cast<u32x4*>(pa.B);
+  if (np == 2) {
+    launch_split<2>(sa, stream, prof);
+    launch_split<2>(sb, stream, prof);
+  } else {
+    launch_split<3>(sa, stream, prof);
+    launch_split<3>(sb, stream, prof);
+  }
+
+  const int nlaunch = sequential ? nchunk : 1;
+  for (int c = 0; c < nlaunch; ++c) {
+    Params q = p;
+    q.kbase = sequential ? c * kF32Chunk : 0;
+    const int klen = (sequential ? std::min(d.K, (c + 1) * kF32Chunk) : d.K) - q.kbase;
+    q.d.K = q.kbase + klen;
+    if (sequential && c > 0) q.d.beta = 1.0;
+    const int ktc = (klen + BKB - 1) / BKB;
+    q.kchunk = ((ktc + splits - 1) / splits) * BKB;
+    q.splits = (klen + q.kchunk - 1) / q.kchunk;
+    q.out = q.splits > 1 ? partials : nullptr;
+    q.work = tiles * q.splits * int64_t(d.batch);
+    q.per = q.work >= 64 ? (q.work + 7) / 8 : 0;
+    const int64_t nblocks = q.per > 0 ? 8 * q.per : q.work;
+    if (nblocks > 0x7fffffff) throw Error(-2, "gemm grid too large");
+    q.fast = 1;
+    q.nfast = (q.tiles_n <= 16 && q.tiles_m >= 4 * q.tiles_n) ? 1 : 0;
+    q.group = (!q.nfast && q.tiles_m > 2 * kTileGroup && q.tiles_n > 2) ? kTileGroup : 0;
+    const dim3 grid{static_cast<unsigned>(nblocks)};
+    const bool timing = prof && prof->on;
+    ProfileRec rec;
+    if (timing) prof->begin(rec, stream);
+    if (BM == 128 && BN == 128) launch_bf16p_cfg<128, 128>(q, pa, np, grid, stream);
+    else if (BM == 128) launch_bf16p_cfg<128, 64>(q, pa, np, grid, stream);
+    else if (BN == 128) launch_bf16p_cfg<64, 128>(q, pa, np, grid, stream);
+    else launch_bf16p_cfg<64, 64>(q, pa, np, grid, stream);
+    UW12G_HIP(hipGetLastError());
+    if (timing)
+      prof->end(rec, stream,
+                "gemm_bf16p_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + std::to_string(np) + ">",
+                2.0 * d.M * double(d.N) * klen * d.batch, 0.0);
+    if (q.splits > 1) {
+      const int64_t MN = int64_t(d.M) * d.N;
+      dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
+      ProfileRec r2;
+      if (timing) prof->begin(r2, stream);
+      hipLaunchKernelGGL(splitk_reduce_kernel, g, dim3(256), 0, stream, q.d, partials, q.splits);
+      UW12G_HIP(hipGetLastError());
+      if (timing)
+        prof->end(r2, stream, "splitk_reduce_kernel", double(q.splits) * d.batch * MN,
+                  8.0 * (q.splits + 1) * d.batch * MN);
+    }
+  }
+  return true;
+}
+
 }  // namespace
 
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
@@ -1224,6 +1709,8 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
     gemm_f64_naive(z, stream);
     return;
   }
+  // BF16 split forms: operands split once into plane tensors when the workspace holds them
+  if (d.prec >= 2 && g_bf16_planes && gemm_bf16_planes(d, stream, ws, prof)) return;
   // coalesce along k only when that index has the unit stride and the other does not
   const bool akc = unit(d.Ak) && !unit(d.Am);
   const bool bkc = unit(d.Bk) && !unit(d.Bn);

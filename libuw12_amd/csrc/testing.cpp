@@ -31,6 +31,12 @@ struct Dev {
 };
 }  // namespace
 
+extern "C" int uw12g_test_set_bf16_planes(int on, long long max_bytes) {
+  g_bf16_planes = on ? 1 : 0;
+  g_plane_bytes_max = max_bytes > 0 ? int64_t(max_bytes) : 0;
+  return 0;
+}
+
 extern "C" long long uw12g_test_map2(uw12g_map2 m, unsigned i) { return (long long)to_map(m).off(i); }
 
 extern "C" int uw12g_test_force_tile(int bm, int bn) {

@@ -122,6 +122,11 @@ struct Workspace;
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof = nullptr);
 // Test hook: a nonzero {BM, BN} forces that tile shape for every launch in the process.
 extern int g_force_tile[2];
+// Test hooks: g_bf16_planes 0 makes the BF16 split forms split their operands
+// inside the GEMM kernel (no plane tensors) for every launch in the process;
+// g_plane_bytes_max > 0 caps the plane workspace of one product (bytes).
+extern int g_bf16_planes;
+extern int64_t g_plane_bytes_max;
 void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
 
 // Device scratch owned by a context: grown on demand, reused across calls.

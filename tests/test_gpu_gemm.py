@@ -173,11 +173,25 @@ def test_fp32_form_within_rounding_bound(name, M, N, K, batch, layout):
     assert np.abs(got - ref).max() > 0
 
 
+@pytest.fixture
+def bf16_path(request):
+    """Operand splitting of the BF16 forms: 'planes' (plane tensors in the
+    workspace, the default), 'inkernel' (split inside the GEMM kernel) or
+    'pieces' (plane tensors with a 256 KiB cap, so products run in pieces)."""
+    mode = getattr(request, 'param', 'planes')
+    L = _lib.load()
+    L.uw12g_test_set_bf16_planes(0 if mode == 'inkernel' else 1, 262144 if mode == 'pieces' else 0)
+    yield mode
+    L.uw12g_test_set_bf16_planes(1, 0)
+
+
 @pytest.mark.parametrize('name,M,N,K,batch', CASES)
 @pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
 @pytest.mark.parametrize('mode', [3, 4])
-def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode):
-    """BF16 split forms (two and three planes) within their error bound; the
+@pytest.mark.parametrize('bf16_path', ['planes', 'inkernel', 'pieces'], indirect=True)
+def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode, bf16_path):
+    """BF16 split forms (two and three planes) within their error bound, with the
+    operands split into plane tensors, inside the kernel, or in pieces; the
     three-plane form is within a small factor of the FP32 form's error."""
     rng = np.random.default_rng(hash(('bf16', name, layout, mode)) % 2**32)
     A = rng.standard_normal(M * K * batch)
@@ -202,7 +216,8 @@ def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode):
 @pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
 @pytest.mark.parametrize('kc', [False, True])
 @pytest.mark.parametrize('n0', [32, 16])
-def test_affine_two_level_k_maps(mode, K, kc, n0):
+@pytest.mark.parametrize('bf16_path', ['planes', 'inkernel'], indirect=True)
+def test_affine_two_level_k_maps(mode, K, kc, n0, bf16_path):
     """Compound k = (r, q) with an inner period n0: with n0 a multiple of the slab
     depth (16 for FP64 / FP32, 32 for the BF16 forms) the unpredicated buffer-load
     path, in both the k-along-waves form (kc False) and the k-along-lanes form (kc
@@ -210,6 +225,8 @@ def test_affine_two_level_k_maps(mode, K, kc, n0):
     the BF16 forms take the general path."""
     if n0 == 16 and mode < 3:
         pytest.skip('n0 = 16 is already slab-affine for the 16-deep slabs')
+    if mode < 3 and bf16_path != 'planes':
+        pytest.skip('operand splitting applies to the BF16 forms only')
     rng = np.random.default_rng(17 + K + 2 * kc + mode + n0)
     M, N = 200, 150
     q = (K + n0 - 1) // n0
@@ -237,11 +254,14 @@ def test_affine_two_level_k_maps(mode, K, kc, n0):
 
 @pytest.mark.parametrize('side', ['A', 'B'])
 @pytest.mark.parametrize('mode', [0, 2, 3, 4])
-def test_operand_span_past_lane_offsets(side, mode):
+@pytest.mark.parametrize('bf16_path', ['planes', 'inkernel'], indirect=True)
+def test_operand_span_past_lane_offsets(side, mode, bf16_path):
     """An operand whose row (A) or column (B) offsets pass 2^28 doubles, the limit of the
     buffer path's 32-bit lane offsets: the launcher splits the output into row / column
     blocks launched from 64-bit base pointers.  The output map has an inner period of 30
     rows / columns, so the blocks must also be whole periods of it."""
+    if mode < 3 and bf16_path != 'planes':
+        pytest.skip('operand splitting applies to the BF16 forms only')
     rng = np.random.default_rng(17 + mode)
     big, K, n_small, stride = 300, 40, 50, 1 << 20
     Abig = np.zeros(big * stride)
