@@ -28,12 +28,14 @@
 //     direct_utils.cpp:206-240, indirect_utils.cpp:261-295).
 //   * In fock_i the per-pair GEMVs J3 * x_jk are batched over j for fixed k
 //     into one GEMM (indirect_utils.cpp:159-191).
-// Parity unpinned: the frozen-core Fock (n_active < n_occ).  The reference
-// forms it as an averaged heuristic (four_electron.cpp:63-76,227-248,
-// direct_utils.cpp:185-203), not as a derivative of the energy, so neither
-// finite differences nor a golden vector pins it; it follows those lines only.
-// Every energy term and the all-electron Fock are pinned (brute-force index
-// sums, long-double evaluation, finite differences of the energy).
+// What pins each function (tests/test_oracle.py): every energy term against
+// brute-force four-index sums and a long-double evaluation; the all-electron
+// Fock against finite differences of the energy; the frozen-core Fock
+// (n_active < n_occ; four_electron.cpp:63-76,227-248, direct_utils.cpp:185-203,
+// indirect_utils.cpp:311-342) against finite differences along orbital paths
+// that move only the active orbitals, of which the reference's F_all + F_act
+// sums are the exact derivative (the same Fock fails along paths that also
+// move the core).
 // BLAS/LAPACK (dgemm_, dsyevd_) are resolved at run time from MKL
 // (libmkl_rt) when present; otherwise a built-in blocked GEMM and a cyclic
 // Jacobi eigensolver are used.
