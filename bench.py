@@ -11,8 +11,9 @@ energies and every accumulation into the Fock matrix).  The plain FP64 path is
 timed beside it on the same inputs, and the mixed result is gated against it:
 `accuracy_gate` reports the largest energy-term and Fock deviations and
 whether both are within 1e-6.  `--form fp64` times the plain path instead;
-`--form bf16x3` / `bf16x6` time the mixed form with its Fock intermediates as
-BF16 split products.  The side forms (FP32, mixed, BF16x3, BF16x6) are timed
+`--form bf16x3` / `bf16x6` / `fp16x3` time the mixed form with its Fock
+intermediates as BF16 or FP16 split products.  The side forms (FP32, mixed,
+BF16x3, BF16x6, FP16x3) are timed
 and compared with FP64 after the timed steps, and so is the differentiation
 form (forward then adjoint, `adjoint_form`).
 
@@ -106,11 +107,15 @@ def roofline_from_profile(records):
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
         # gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>: PREC 1 runs on the FP32 matrix cores.
-        # gemm_bf16x_kernel<..., planes>: 3 (two planes) or 6 (three planes) BF16 products per
-        # multiply-add, so the peak of the algorithmic FLOP rate is the BF16 peak over that count.
-        last = dom.rstrip('>').rsplit(',', 1)[-1]
+        # gemm_bf16x_kernel<..., planes> and gemm_planes_kernel<BM, BN, planes, half>: 3 (two
+        # planes) or 6 (three planes) BF16 / FP16 products per multiply-add, so the peak of the
+        # algorithmic FLOP rate is the BF16 (= FP16) dense peak over that count.
+        args_ = dom.rstrip('>').split('<', 1)[-1].split(',')
+        last = args_[-1]
         if dom.startswith('gemm_bf16x_kernel<'):
             peak = round(PEAK_BF16_TFLOPS / (3 if last == '2' else 6), 1)
+        elif dom.startswith('gemm_planes_kernel<'):
+            peak = round(PEAK_BF16_TFLOPS / (3 if args_[2] == '2' else 6), 1)
         else:
             peak = PEAK_FP32_TFLOPS if dom.startswith('gemm_f64_kernel<') and last == '1' else PEAK_FP64_TFLOPS
         roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=peak, unit='TFLOP/s',
@@ -361,7 +366,8 @@ def measure(name, system, handles, ctx, kappa, form, steps, warmup, side_forms, 
 
 
 def PRECISION(u):
-    return {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED, 'bf16x3': u.MIXED_BF16X3, 'bf16x6': u.MIXED_BF16X6}
+    return {'fp64': u.FP64, 'fp32': u.FP32, 'mixed': u.MIXED, 'bf16x3': u.MIXED_BF16X3, 'bf16x6': u.MIXED_BF16X6,
+            'fp16x3': u.MIXED_FP16X3}
 
 
 _GATE_TEXT = 'gated at <=1e-6 Eh per term and <=1e-6 Fock max-abs against the plain FP64 path of the same run'
@@ -373,7 +379,10 @@ FORM_TEXT = {'fp64': 'plain FP64',
              'bf16x3': 'mixed form with the Fock intermediates as BF16x3 split products (two bfloat16 planes per '
                        'operand, three products, FP32 sums) on BF16 matrix cores, ' + _GATE_TEXT,
              'bf16x6': 'mixed form with the Fock intermediates as BF16x6 split products (three bfloat16 planes per '
-                       'operand, six products, FP32 sums) on BF16 matrix cores, ' + _GATE_TEXT}
+                       'operand, six products, FP32 sums) on BF16 matrix cores, ' + _GATE_TEXT,
+             'fp16x3': 'mixed form with the Fock intermediates as FP16x3 split products (operand rows scaled by '
+                       'powers of two, two IEEE half planes per operand, three products, FP32 sums) on FP16 matrix '
+                       'cores, ' + _GATE_TEXT}
 INTEGRALS_TEXT = {'host': 'host integral arrays copied to HBM',
                   'device': 'integrals generated on the device, stored in HBM',
                   'streamed': 'integrals generated on the device, not stored: every evaluation regenerates its '
@@ -429,7 +438,7 @@ def main():
     ap.add_argument('--kappa', default=None, help='k0,k1 (default: (1,0.5) for c1/c2, (1,0) for c3-c5)')
     ap.add_argument('--integrals', default=None, choices=['host', 'device', 'streamed'],
                     help='default: host for c1/c2, device for c3/c4, streamed for c5')
-    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32', 'bf16x3', 'bf16x6'],
+    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32', 'bf16x3', 'bf16x6', 'fp16x3'],
                     help='precision form of the timed steps (the plain FP64 path is run beside it)')
     ap.add_argument('--side-reps', type=int, default=3,
                     help='timed evaluations of each side form (after one untimed one when more than 1)')
@@ -504,7 +513,7 @@ def main():
         'scaling': 'weak',
         'vs_baseline': None,
         'dtype': {'fp64': 'f64', 'fp32': 'f32', 'mixed': 'f32+f64', 'bf16x3': 'bf16x3+f64',
-                  'bf16x6': 'bf16x6+f64'}[args.form],
+                  'bf16x6': 'bf16x6+f64', 'fp16x3': 'fp16x3+f64'}[args.form],
         'data': 'synthetic (seeded s-Gaussian cluster, analytic Boys-F0 integrals)',
         'config': {'workload': workload(args.config, system, kappa, args.form, integrals),
                    'form': args.form, 'kappa': list(kappa), 'integrals': integrals, 'parallelism': 'single-gpu'},

@@ -43,13 +43,17 @@ typedef struct uw12g_abs_projectors uw12g_abs_projectors;
  * are the mixed form with those intermediates as BF16 split products on the
  * BF16 matrix cores (FP32 sums as above): each FP64 operand split into two
  * bfloat16 planes and three plane products (about 3 * 2^-16 relative error
- * per product), or into three planes and six products (about 2^-22). */
+ * per product), or into three planes and six products (about 2^-22).
+ * UW12G_MIXED_FP16X3 splits each operand row, scaled by a power of two, into
+ * two IEEE half planes and runs three products on the FP16 matrix cores
+ * (about 3 * 2^-22 per product at the cost of BF16x3). */
 typedef enum {
   UW12G_FP64 = 0,
   UW12G_FP32 = 1,
   UW12G_MIXED = 2,
   UW12G_MIXED_BF16X3 = 3,
-  UW12G_MIXED_BF16X6 = 4
+  UW12G_MIXED_BF16X6 = 4,
+  UW12G_MIXED_FP16X3 = 5
 } uw12g_precision;
 
 /* Term order of terms_out[] */

@@ -14,7 +14,7 @@ from ._lib import UW12Error, check, dp, szp, vp
 
 TERMS = ('e2_direct', 'e2_indirect', 'e3_direct', 'e3_indirect', 'e4_direct', 'e4_indirect')
 # uw12g_precision values (include/uw12g.h)
-FP64, FP32, MIXED, MIXED_BF16X3, MIXED_BF16X6 = 0, 1, 2, 3, 4
+FP64, FP32, MIXED, MIXED_BF16X3, MIXED_BF16X6, MIXED_FP16X3 = 0, 1, 2, 3, 4, 5
 
 
 def _f64(a):

@@ -17,6 +17,7 @@
 #include <string>
 #include <type_traits>
 
+#include "kernels.h"
 #include "uw12g_internal.h"
 
 namespace uw12g {
@@ -545,6 +546,49 @@ __device__ inline void split_pair(double v0, double v1, uint32_t (&w)[NP]) {
   }
 }
 
+// FP16 split form (prec 4): each operand row is scaled by a power of two that
+// puts its largest magnitude in [2^14, 2^15) (row_scale_kernel), then split
+// into two IEEE half planes a s ~ h + l (h the round-to-nearest half of a s,
+// l that of the residual, formed exactly in FP64).  With 11 significant bits
+// per plane the three products hh + hl + lh on v_mfma_f32_16x16x32_f16 carry
+// a relative error of about 3 * 2^-22 per product (against 3 * 2^-16 for two
+// bfloat16 planes) at the same matrix-core cost; the output is scaled back
+// by 1 / (s_row s_col) in the epilogue (exact).  Elements more than 2^18 below
+// their row's maximum lose low bits of l to the half subnormal range: their
+// absolute error stays below 2^-40 of the row maximum.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+__device__ inline uint32_t pack_f16(float a, float b) {
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(fl2{a, b}, h2));
+}
+__device__ inline double f16_lo(uint32_t w) { return double(__builtin_bit_cast(_Float16, uint16_t(w & 0xffffu))); }
+__device__ inline double f16_hi(uint32_t w) { return double(__builtin_bit_cast(_Float16, uint16_t(w >> 16))); }
+
+// Plane words of two consecutive (already scaled) k values: h = half(v) (via
+// FP32: the double rounding leaves |v - h| within one half ulp of h, and
+// v - h is exact in FP64), l = half(v - h).
+__device__ inline void split_pair_f16(double v0, double v1, uint32_t (&w)[2]) {
+  const uint32_t h = pack_f16(float(v0), float(v1));
+  w[0] = h;
+  w[1] = pack_f16(float(v0 - f16_lo(h)), float(v1 - f16_hi(h)));
+}
+
+// Power-of-two scale s of a row whose largest magnitude is m: m s in
+// [2^14, 2^15) (an all-zero row keeps s = 1); inv = 1 / s, exact.
+__device__ inline void pow2_scale(double m, double& s, double& inv) {
+  if (!(m > 0.0) || !(m < 1e300)) {
+    s = 1.0;
+    inv = 1.0;
+    return;
+  }
+  int E = 0;
+  (void)frexp(m, &E);  // m = f 2^E, f in [0.5, 1)
+  const int e = min(max(15 - E, -1000), 1000);
+  s = ldexp(1.0, e);
+  inv = ldexp(1.0, -e);
+}
+
 template <int BM, int BN, bool AKC, bool BKC, bool KS, int NP>
 __global__ __launch_bounds__(NT) void gemm_bf16x_kernel(Params p) {
   constexpr int WM = BM / 2, WN = BN / 2;
@@ -886,7 +930,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16x_kernel(Params p) {
 // ---------------------------------------------------------------------------
 // BF16 plane tensors.  For the split forms the operands are split once per
 // product into packed bfloat16 planes (split_planes_kernel), so that the GEMM
-// (gemm_bf16p_kernel) reads 2 bytes per element and plane instead of the 8 of
+// (gemm_planes_kernel) reads 2 bytes per element and plane instead of the 8 of
 // the FP64 value, with no conversion in its main loop.  Layout of an operand
 // viewed as X(r, k) (r: the output row for A, the output column for B) over
 // nb batches: 16-byte units holding k = 8 q .. 8 q + 7 of one r in one plane,
@@ -902,10 +946,12 @@ struct PlaneDesc {
   int64_t Kb = 0;
   int R = 0, K = 0, nb = 1, KQ = 0, Rp = 0;
   u32x4* out = nullptr;
+  const double* scale = nullptr;  // FP16 form: power-of-two scale of row (b, r) at b Rp + r
 };
 
 // One thread per unit, r fastest (the unit writes of a wave are contiguous).
-template <int NP>
+// F16: two half planes of the scaled values (NP = 2); else NP bfloat16 planes.
+template <int NP, bool F16>
 __global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s) {
   const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
   const int64_t plane = units;
@@ -924,26 +970,155 @@ __global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s) {
       if (s.ks) x *= s.ks[ok ? s.Kb * b + s.Km.off(k) : 0];
       v[e] = ok ? x : 0.0;
     }
+    if constexpr (F16) {
+      const double sc = s.scale[int64_t(b) * s.Rp + r];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) v[e] *= sc;
+    }
     uint32_t w[4][NP];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (F16) {
+        static_assert(NP == 2, "the FP16 form has two planes");
+        uint32_t t2[2];
+        split_pair_f16(v[2 * i], v[2 * i + 1], t2);
+        w[i][0] = t2[0];
+        w[i][1] = t2[1];
+      } else {
+        split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+      }
+    }
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) s.out[pl * plane + u] = u32x4{w[0][pl], w[1][pl], w[2][pl], w[3][pl]};
   }
+}
+
+// The same units for operands with k contiguous in memory and the rows strided
+// (the per-unit kernel above would read 64 scattered 8-byte words per wave and
+// load): a workgroup stages a tile of 64 rows x 32 k through LDS, loading along
+// k (32 consecutive k of one row per 32 lanes) and writing the units along the
+// rows (64 consecutive units per wave and plane).
+template <int NP, bool F16>
+__global__ __launch_bounds__(256) void split_planes_kc_kernel(PlaneDesc s) {
+  __shared__ double T[64][33];
+  const int64_t plane = int64_t(s.nb) * s.KQ * s.Rp;
+  const int64_t rtiles = s.Rp / 64;  // Rp is a multiple of 128
+  const int qtiles = s.KQ / 4;       // KQ = 4 ceil(K / 32)
+  const int64_t tiles = int64_t(s.nb) * qtiles * rtiles;
+  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+    const int r0 = int(tile % rtiles) * 64;
+    const int64_t rest = tile / rtiles;
+    const int kt = int(rest % qtiles), b = int(rest / qtiles);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int idx = int(threadIdx.x) + 256 * i;
+      const int row = idx >> 5, kk = idx & 31;
+      const int r = r0 + row, k = 32 * kt + kk;
+      double x = 0.0;
+      if (r < s.R && k < s.K) {
+        x = s.X[s.Xb * b + s.Xr.off(r) + s.Xk.off(k)];
+        if (s.ks) x *= s.ks[s.Kb * b + s.Km.off(k)];
+        if constexpr (F16) x *= s.scale[int64_t(b) * s.Rp + r];
+      }
+      T[row][kk] = x;
+    }
+    __syncthreads();
+    const int row = int(threadIdx.x & 63), qq = int(threadIdx.x >> 6);
+    double v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = T[row][8 * qq + e];
+    uint32_t w[4][NP];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if constexpr (F16) {
+        uint32_t t2[2];
+        split_pair_f16(v[2 * i], v[2 * i + 1], t2);
+        w[i][0] = t2[0];
+        w[i][1] = t2[1];
+      } else {
+        split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+      }
+    }
+    const int64_t u = (int64_t(b) * s.KQ + 4 * kt + qq) * s.Rp + r0 + row;
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl) s.out[pl * plane + u] = u32x4{w[0][pl], w[1][pl], w[2][pl], w[3][pl]};
+    __syncthreads();
+  }
+}
+
+// Row scales of the FP16 form: for every row (b, r), r < Rp, of an operand the
+// power of two of pow2_scale from max_k |X(r, k) ks(k)| (padding rows: 1), in
+// two steps.  row_max_kernel: the k range is cut into kparts chunks so that
+// operands with few rows still fill the chip; each thread (KC: each wave, with
+// the lanes along k, when k is contiguous in memory) takes the maximum of one
+// row over one chunk and folds it into mx[row] with an atomic maximum on the
+// bit pattern (ordered like the value for non-negative doubles; mx starts at
+// zero).  scale_from_max_kernel then turns mx[row] into the scale and its
+// inverse, in place.
+template <bool KC>
+__global__ __launch_bounds__(256) void row_max_kernel(PlaneDesc s, int kparts, int kchunk, unsigned long long* mx) {
+  const int64_t rows = int64_t(s.nb) * s.Rp;
+  const int64_t gt = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  const int64_t w = KC ? gt / 64 : gt;
+  const int lane = KC ? int(threadIdx.x & 63) : 0;
+  if (w >= rows * kparts) return;  // wave-uniform when KC (the grid covers whole waves)
+  const int64_t u = w % rows;
+  const int part = int(w / rows);
+  const int r = int(u % s.Rp), b = int(u / s.Rp);
+  if (r >= s.R) return;
+  const int k0 = part * kchunk, k1 = min(s.K, k0 + kchunk);
+  const int64_t base = s.Xb * b + s.Xr.off(r);
+  double m = 0.0;
+  for (int k = k0 + lane; k < k1; k += (KC ? 64 : 1)) {
+    double x = s.X[base + s.Xk.off(k)];
+    if (s.ks) x *= s.ks[s.Kb * b + s.Km.off(k)];
+    m = fmax(m, fabs(x));
+  }
+  if constexpr (KC) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+    if (lane != 0) return;
+  }
+  if (m > 0.0) atomicMax(mx + u, static_cast<unsigned long long>(__double_as_longlong(m)));
+}
+
+__global__ __launch_bounds__(256) void scale_from_max_kernel(int64_t rows, double* scale, double* inv) {
+  const int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (u >= rows) return;
+  double sc, iv;
+  pow2_scale(scale[u], sc, iv);  // scale[u] holds the row maximum (the bits of mx)
+  scale[u] = sc;
+  inv[u] = iv;
+}
+
+// Row scales of the FP16 form from caller-supplied bounds on the row maxima:
+// row (b, r) gets pow2_scale(hint[b hb + r]) (padding rows: 1).
+__global__ __launch_bounds__(256) void hint_scale_kernel(const double* hint, int64_t hb, int R, int Rp, int nb,
+                                                         double* scale, double* inv) {
+  const int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (u >= int64_t(nb) * Rp) return;
+  const int r = int(u % Rp), b = int(u / Rp);
+  double sc = 1.0, iv = 1.0;
+  if (r < R) pow2_scale(hint[hb * b + r], sc, iv);
+  scale[u] = sc;
+  inv[u] = iv;
 }
 
 struct PlaneArgs {
   const u32x4* A = nullptr;
   const u32x4* B = nullptr;
   int nbA = 1, nbB = 1, RpA = 0, RpB = 0, KQ = 0;
+  const double* invA = nullptr;  // FP16 form: 1 / row scale of A (b RpA + m) and of B (b RpB + n)
+  const double* invB = nullptr;
 };
 
 // GEMM over plane tensors: the tile order, split-K partials and epilogue of
 // gemm_bf16x_kernel; k slabs of 32 loaded as 16-byte units (4 BM / 256 per
 // thread and plane) into registers two slabs ahead, written to the same LDS
-// layout, the same MFMA plane products.
-template <int BM, int BN, int NP>
-__global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) {
+// layout, the same MFMA plane products.  F16: the planes are IEEE halves
+// (v_mfma_f32_16x16x32_f16) and the epilogue undoes the row scales.
+template <int BM, int BN, int NP, bool F16>
+__global__ __launch_bounds__(NT) void gemm_planes_kernel(Params p, PlaneArgs pa) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int UA = 4 * BM / NT, UB = 4 * BN / NT;  // units per thread, plane and slab
@@ -1029,29 +1204,35 @@ __global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) 
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
   auto compute = [&]() {
-    bf8 bfr[NB][NP];
+    u32x4 bfr[NB][NP];
 #pragma unroll
     for (int j = 0; j < NB; ++j)
 #pragma unroll
-      for (int pl = 0; pl < NP; ++pl)
-        bfr[j][pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const u32x4*>(&Bs[pl][wn0 + 16 * j + lr][4 * lk]));
+      for (int pl = 0; pl < NP; ++pl) bfr[j][pl] = *reinterpret_cast<const u32x4*>(&Bs[pl][wn0 + 16 * j + lr][4 * lk]);
+    // one plane product c += b a (bfloat16 or half operands)
+    auto mma = [](const u32x4& b, const u32x4& a, f4 c) {
+      if constexpr (F16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, b), __builtin_bit_cast(h8, a), c, 0, 0, 0);
+      else
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, b), __builtin_bit_cast(bf8, a), c, 0, 0,
+                                                       0);
+    };
 #pragma unroll
     for (int i = 0; i < MB; ++i) {
-      bf8 afr[NP];
+      u32x4 afr[NP];
 #pragma unroll
-      for (int pl = 0; pl < NP; ++pl)
-        afr[pl] = __builtin_bit_cast(bf8, *reinterpret_cast<const u32x4*>(&As[pl][wm0 + 16 * i + lr][4 * lk]));
+      for (int pl = 0; pl < NP; ++pl) afr[pl] = *reinterpret_cast<const u32x4*>(&As[pl][wm0 + 16 * i + lr][4 * lk]);
 #pragma unroll
       for (int j = 0; j < NB; ++j) {
         f4 c = acc[i][j];
         if constexpr (NP == 3) {
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][2], afr[0], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[2], c, 0, 0, 0);
-          c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[1], c, 0, 0, 0);
+          c = mma(bfr[j][2], afr[0], c);
+          c = mma(bfr[j][0], afr[2], c);
+          c = mma(bfr[j][1], afr[1], c);
         }
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][1], afr[0], c, 0, 0, 0);
-        c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[1], c, 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(bfr[j][0], afr[0], c, 0, 0, 0);
+        c = mma(bfr[j][1], afr[0], c);
+        c = mma(bfr[j][0], afr[1], c);
+        acc[i][j] = mma(bfr[j][0], afr[0], c);
       }
     }
   };
@@ -1096,18 +1277,30 @@ __global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) 
   }
 
   auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + 4 * lk + r; };
+  // FP16: alpha / (s_row s_col), a power of two times alpha
+  const double* invA = F16 ? pa.invA + int64_t(pa.nbA > 1 ? b : 0) * pa.RpA : nullptr;
+  const double* invB = F16 ? pa.invB + int64_t(pa.nbB > 1 ? b : 0) * pa.RpB : nullptr;
+  double colf[NB][4];
+#pragma unroll
+  for (int j = 0; j < NB; ++j)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int n = ncol(j, r);
+      colf[j][r] = F16 ? (n < d.N ? invB[n] : 0.0) * d.alpha : d.alpha;
+    }
   if (p.out) {
     double* W = p.out + (int64_t(split) * d.batch + b) * int64_t(d.M) * d.N;
 #pragma unroll
     for (int i = 0; i < MB; ++i) {
       const int m = m0 + wm0 + 16 * i + lr;
       if (m >= d.M) continue;
+      const double rf = F16 ? invA[m] : 1.0;
 #pragma unroll
       for (int j = 0; j < NB; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = ncol(j, r);
-          if (n < d.N) W[m + int64_t(d.M) * n] = d.alpha * double(acc[i][j][r]);
+          if (n < d.N) W[m + int64_t(d.M) * n] = colf[j][r] * rf * double(acc[i][j][r]);
         }
     }
     return;
@@ -1118,6 +1311,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) 
     const int m = m0 + wm0 + 16 * i + lr;
     if (m >= d.M) continue;
     const int64_t mo = d.Cm.off(m);
+    const double rf = F16 ? invA[m] : 1.0;
 #pragma unroll
     for (int j = 0; j < NB; ++j)
 #pragma unroll
@@ -1125,7 +1319,7 @@ __global__ __launch_bounds__(NT) void gemm_bf16p_kernel(Params p, PlaneArgs pa) 
         const int n = ncol(j, r);
         if (n >= d.N) continue;
         double* c = Cp + mo + d.Cn.off(n);
-        const double v = d.alpha * double(acc[i][j][r]);
+        const double v = colf[j][r] * rf * double(acc[i][j][r]);
         *c = d.beta == 0.0 ? v : v + d.beta * *c;
       }
   }
@@ -1473,41 +1667,93 @@ constexpr int64_t kPlaneBytesMax = int64_t(16) << 30;
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
 
-template <int NP>
+// Plane kinds (GemmDesc::prec of the split forms): 2 two bfloat16 planes, 3
+// three bfloat16 planes, 4 two half planes of row-scaled values.
+constexpr int planes_of(int prec) { return prec == 3 ? 3 : 2; }
+
+template <int NP, bool F16>
 void launch_split(const PlaneDesc& s, hipStream_t stream, Profiler* prof) {
   const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
   const unsigned grid = unsigned(std::min<int64_t>((units + 255) / 256, 8192));
   const bool timing = prof && prof->on;
   ProfileRec rec;
   if (timing) prof->begin(rec, stream);
-  hipLaunchKernelGGL(split_planes_kernel<NP>, dim3(grid), dim3(256), 0, stream, s);
+  if (s.Xk.s0 == 1 && s.Xr.s0 != 1) {
+    const int64_t tiles = int64_t(s.nb) * (s.KQ / 4) * (s.Rp / 64);
+    const unsigned g = unsigned(std::min<int64_t>(tiles, 16384));
+    hipLaunchKernelGGL((split_planes_kc_kernel<NP, F16>), dim3(g), dim3(256), 0, stream, s);
+  } else {
+    hipLaunchKernelGGL((split_planes_kernel<NP, F16>), dim3(grid), dim3(256), 0, stream, s);
+  }
   UW12G_HIP(hipGetLastError());
   if (timing)
-    prof->end(rec, stream, "split_planes_kernel<" + std::to_string(NP) + ">", 0.0,
+    prof->end(rec, stream, "split_planes_kernel<" + std::to_string(NP) + "," + (F16 ? "1" : "0") + ">", 0.0,
               (8.0 + 2.0 * NP) * double(s.R) * s.K * s.nb);
 }
 
-template <int BM, int BN>
-void launch_bf16p_cfg(const Params& p, const PlaneArgs& pa, int np, dim3 grid, hipStream_t s) {
-  if (np == 2) hipLaunchKernelGGL((gemm_bf16p_kernel<BM, BN, 2>), grid, dim3(NT), 0, s, p, pa);
-  else hipLaunchKernelGGL((gemm_bf16p_kernel<BM, BN, 3>), grid, dim3(NT), 0, s, p, pa);
+// Row scales of one operand (FP16 form) from bounds on its row maxima.
+void launch_hint_scale(const PlaneDesc& s, const double* hint, int64_t hb, double* scale, double* inv,
+                       hipStream_t stream) {
+  const int64_t rows = int64_t(s.nb) * s.Rp;
+  hipLaunchKernelGGL(hint_scale_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, stream, hint, hb, s.R, s.Rp,
+                     s.nb, scale, inv);
+  UW12G_HIP(hipGetLastError());
+}
+
+// Row scales of one operand (FP16 form): scale and inverse per row (b, r), r < Rp.
+void launch_row_scale(const PlaneDesc& s, double* scale, double* inv, hipStream_t stream, Profiler* prof) {
+  const int64_t rows = int64_t(s.nb) * s.Rp;
+  const bool kc = s.Xk.s0 == 1 && s.Xr.s0 != 1;  // k contiguous: a wave per row and chunk
+  // chunks of k: about 2^19 threads (KC: 2^13 waves) in all, chunks of at least 64 k
+  const int64_t want = kc ? (int64_t(1) << 13) : (int64_t(1) << 19);
+  const int kparts = int(std::max<int64_t>(1, std::min<int64_t>((want + rows - 1) / rows, (s.K + 63) / 64)));
+  const int kchunk = (s.K + kparts - 1) / kparts;
+  const int64_t threads = rows * kparts * (kc ? 64 : 1);
+  const int64_t blocks = (threads + 255) / 256;
+  if (blocks > 0x7fffffff) throw Error(-2, "row scale grid too large");
+  const bool timing = prof && prof->on;
+  ProfileRec rec;
+  if (timing) prof->begin(rec, stream);
+  fill(scale, rows, 0.0, stream);
+  unsigned long long* mx = reinterpret_cast<unsigned long long*>(scale);
+  if (kc)
+    hipLaunchKernelGGL(row_max_kernel<true>, dim3(unsigned(blocks)), dim3(256), 0, stream, s, kparts, kchunk, mx);
+  else
+    hipLaunchKernelGGL(row_max_kernel<false>, dim3(unsigned(blocks)), dim3(256), 0, stream, s, kparts, kchunk, mx);
+  UW12G_HIP(hipGetLastError());
+  hipLaunchKernelGGL(scale_from_max_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, stream, rows, scale, inv);
+  UW12G_HIP(hipGetLastError());
+  if (timing) prof->end(rec, stream, "row_max_kernel", 0.0, 8.0 * double(s.R) * s.K * s.nb);
+}
+
+template <int BM, int BN, int NP, bool F16>
+const void* planes_kernel_ptr() {
+  return reinterpret_cast<const void*>(&gemm_planes_kernel<BM, BN, NP, F16>);
 }
 
 template <int BM, int BN>
-int64_t bf16p_slots(int np) {
+void launch_planes_cfg(const Params& p, const PlaneArgs& pa, int kind, dim3 grid, hipStream_t s) {
+  if (kind == 2) hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, 2, false>), grid, dim3(NT), 0, s, p, pa);
+  else if (kind == 3) hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, 3, false>), grid, dim3(NT), 0, s, p, pa);
+  else hipLaunchKernelGGL((gemm_planes_kernel<BM, BN, 2, true>), grid, dim3(NT), 0, s, p, pa);
+}
+
+template <int BM, int BN>
+int64_t planes_slots(int kind) {
   static std::mutex mu;
   static std::map<int, int64_t> cache;
   std::lock_guard<std::mutex> lock(mu);
-  auto it = cache.find(np);
+  auto it = cache.find(kind);
   if (it != cache.end()) return it->second;
-  const void* f = np == 2 ? reinterpret_cast<const void*>(&gemm_bf16p_kernel<BM, BN, 2>)
-                          : reinterpret_cast<const void*>(&gemm_bf16p_kernel<BM, BN, 3>);
+  const void* f = kind == 2   ? planes_kernel_ptr<BM, BN, 2, false>()
+                  : kind == 3 ? planes_kernel_ptr<BM, BN, 3, false>()
+                              : planes_kernel_ptr<BM, BN, 2, true>();
   int per_cu = 0, dev = 0, cus = 0;
   UW12G_HIP(hipGetDevice(&dev));
   UW12G_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   UW12G_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, NT, 0));
   const int64_t slots = int64_t(std::max(per_cu, 1)) * std::max(cus, 1);
-  cache[np] = slots;
+  cache[kind] = slots;
   return slots;
 }
 
@@ -1516,7 +1762,8 @@ int64_t bf16p_slots(int np) {
 // in-kernel split of gemm_bf16x_kernel).  Large products run in pieces.
 bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   if (!ws || ws->carve) return false;
-  const int np = d.prec == 2 ? 2 : 3;
+  const int kind = d.prec, np = planes_of(d.prec);
+  const bool f16 = kind == 4;
   const int nbA = (d.Ab == 0 && (!d.ks || d.Kb == 0)) ? 1 : d.batch;
   const int nbB = d.Bb == 0 ? 1 : d.batch;
   const int64_t KQ = 4 * ((int64_t(d.K) + BKB - 1) / BKB);
@@ -1548,8 +1795,8 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
   if (tiles > 0x7fffffff) return false;
   const int kt = int(KQ / 4);
-  const int64_t slots = BM == 128 ? (BN == 128 ? bf16p_slots<128, 128>(np) : bf16p_slots<128, 64>(np))
-                                  : (BN == 128 ? bf16p_slots<64, 128>(np) : bf16p_slots<64, 64>(np));
+  const int64_t slots = BM == 128 ? (BN == 128 ? planes_slots<128, 128>(kind) : planes_slots<128, 64>(kind))
+                                  : (BN == 128 ? planes_slots<64, 128>(kind) : planes_slots<64, 64>(kind));
   int splits = kt >= 16 ? choose_splits(tiles * d.batch, kt, d.K, d.batch, slots) : 1;
   // FP32 sums over at most kF32Chunk k values: that many chunks at least
   const int nchunk = (d.K + kF32Chunk - 1) / kF32Chunk;
@@ -1560,7 +1807,9 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   }
   if (sequential) splits = 1;
   const int64_t part = splits > 1 ? int64_t(splits) * d.batch * d.M * d.N * 8 : 0;
-  const int64_t need = round_up(bytesA, 256) + round_up(bytesB, 256) + part;
+  // FP16: scale and inverse scale per row of A and of B
+  const int64_t bytesS = f16 ? round_up(16 * (int64_t(nbA) * RpA), 256) + round_up(16 * (int64_t(nbB) * RpB), 256) : 0;
+  const int64_t need = round_up(bytesA, 256) + round_up(bytesB, 256) + bytesS + part;
 
   // workspace budget: what it holds already, or half of the free device memory
   size_t free_b = 0, total_b = 0;
@@ -1579,6 +1828,8 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
         e.B = d.B + d.Bb * b0;
         e.C = d.C + d.Cb * b0;
         if (d.ks) e.ks = d.ks + d.Kb * b0;
+        if (d.hintA) e.hintA = d.hintA + d.hAb * b0;
+        if (d.hintB) e.hintB = d.hintB + d.hBb * b0;
         if (!gemm_bf16_planes(e, stream, ws, prof)) gemm_f64(e, stream, nullptr, prof);
       }
       return true;
@@ -1598,10 +1849,12 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
         e.M = len;
         e.A = d.A + d.Am.off(uint32_t(x0));
         e.C = d.C + d.Cm.off(uint32_t(x0));
+        if (d.hintA) e.hintA = d.hintA + x0;
       } else {
         e.N = len;
         e.B = d.B + d.Bn.off(uint32_t(x0));
         e.C = d.C + d.Cn.off(uint32_t(x0));
+        if (d.hintB) e.hintB = d.hintB + x0;
       }
       if (!gemm_bf16_planes(e, stream, ws, prof)) gemm_f64(e, stream, nullptr, prof);
     }
@@ -1612,7 +1865,14 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   PlaneArgs pa;
   pa.A = reinterpret_cast<const u32x4*>(base);
   pa.B = reinterpret_cast<const u32x4*>(base + round_up(bytesA, 256));
-  double* partials = part ? reinterpret_cast<double*>(base + round_up(bytesA, 256) + round_up(bytesB, 256)) : nullptr;
+  char* sbase = base + round_up(bytesA, 256) + round_up(bytesB, 256);
+  double* partials = part ? reinterpret_cast<double*>(sbase + bytesS) : nullptr;
+  double* scA = f16 ? reinterpret_cast<double*>(sbase) : nullptr;
+  double* scB = f16 ? reinterpret_cast<double*>(sbase + round_up(16 * (int64_t(nbA) * RpA), 256)) : nullptr;
+  if (f16) {
+    pa.invA = scA + int64_t(nbA) * RpA;
+    pa.invB = scB + int64_t(nbB) * RpB;
+  }
   pa.nbA = nbA;
   pa.nbB = nbB;
   pa.RpA = int(RpA);
@@ -1643,12 +1903,21 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   sb.KQ = int(KQ);
   sb.Rp = int(RpB);
   sb.out = const_cast<u32x4*>(pa.B);
-  if (np == 2) {
-    launch_split<2>(sa, stream, prof);
-    launch_split<2>(sb, stream, prof);
+  if (f16) {
+    if (d.hintA) launch_hint_scale(sa, d.hintA, d.hAb, scA, const_cast<double*>(pa.invA), stream);
+    else launch_row_scale(sa, scA, const_cast<double*>(pa.invA), stream, prof);
+    if (d.hintB) launch_hint_scale(sb, d.hintB, d.hBb, scB, const_cast<double*>(pa.invB), stream);
+    else launch_row_scale(sb, scB, const_cast<double*>(pa.invB), stream, prof);
+    sa.scale = scA;
+    sb.scale = scB;
+    launch_split<2, true>(sa, stream, prof);
+    launch_split<2, true>(sb, stream, prof);
+  } else if (np == 2) {
+    launch_split<2, false>(sa, stream, prof);
+    launch_split<2, false>(sb, stream, prof);
   } else {
-    launch_split<3>(sa, stream, prof);
-    launch_split<3>(sb, stream, prof);
+    launch_split<3, false>(sa, stream, prof);
+    launch_split<3, false>(sb, stream, prof);
   }
 
   const int nlaunch = sequential ? nchunk : 1;
@@ -1673,14 +1942,15 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
     const bool timing = prof && prof->on;
     ProfileRec rec;
     if (timing) prof->begin(rec, stream);
-    if (BM == 128 && BN == 128) launch_bf16p_cfg<128, 128>(q, pa, np, grid, stream);
-    else if (BM == 128) launch_bf16p_cfg<128, 64>(q, pa, np, grid, stream);
-    else if (BN == 128) launch_bf16p_cfg<64, 128>(q, pa, np, grid, stream);
-    else launch_bf16p_cfg<64, 64>(q, pa, np, grid, stream);
+    if (BM == 128 && BN == 128) launch_planes_cfg<128, 128>(q, pa, kind, grid, stream);
+    else if (BM == 128) launch_planes_cfg<128, 64>(q, pa, kind, grid, stream);
+    else if (BN == 128) launch_planes_cfg<64, 128>(q, pa, kind, grid, stream);
+    else launch_planes_cfg<64, 64>(q, pa, kind, grid, stream);
     UW12G_HIP(hipGetLastError());
     if (timing)
       prof->end(rec, stream,
-                "gemm_bf16p_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + std::to_string(np) + ">",
+                "gemm_planes_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + std::to_string(np) + "," +
+                    (f16 ? "1" : "0") + ">",
                 2.0 * d.M * double(d.N) * klen * d.batch, 0.0);
     if (q.splits > 1) {
       const int64_t MN = int64_t(d.M) * d.N;
@@ -1711,6 +1981,14 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   }
   // BF16 split forms: operands split once into plane tensors when the workspace holds them
   if (d.prec >= 2 && g_bf16_planes && gemm_bf16_planes(d, stream, ws, prof)) return;
+  if (d.prec == 4) {
+    // the FP16 form needs its row scales and planes in the workspace; without one the
+    // product runs as the in-kernel split with three bfloat16 planes (2^-22 per product)
+    GemmDesc e = d;
+    e.prec = 3;
+    gemm_f64(e, stream, ws, prof);
+    return;
+  }
   // coalesce along k only when that index has the unit stride and the other does not
   const bool akc = unit(d.Ak) && !unit(d.Am);
   const bool bkc = unit(d.Bk) && !unit(d.Bn);
@@ -1733,6 +2011,8 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
           c.A = d.A + d.Am.off(uint32_t(m0));
           c.B = d.B + d.Bn.off(uint32_t(n0));
           c.C = d.C + d.Cm.off(uint32_t(m0)) + d.Cn.off(uint32_t(n0));
+          if (d.hintA) c.hintA = d.hintA + m0;
+          if (d.hintB) c.hintB = d.hintB + n0;
           gemm_f64(c, stream, ws, prof);
         }
       return;

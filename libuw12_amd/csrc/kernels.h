@@ -60,5 +60,11 @@ void scale_rows(const double* in, int64_t rows, int64_t ni, int64_t nb, int64_t 
 // = in[r + rows (j + a k)] with j = j0 + jj, k = kk (kk < c) or kk + j0, nk = c + a - j0.
 void pair_gather(const double* in, int64_t rows, int a, int c, int j0, int nj, int nk, double* out, hipStream_t s);
 void fill(double* p, int64_t n, double v, hipStream_t s);
+// out[r] = || x[r + stride t], t < nt ||_2 for r < rows.
+void row_norms(const double* x, int64_t rows, int64_t stride, int nt, double* out, hipStream_t s);
+// x column-major n0 x n1: out[i] = max_j x[i + n0 j] (over_cols) or out[j] = max_i x[i + n0 j].
+void max_over(const double* x, int64_t n0, int64_t n1, bool over_cols, double* out, hipStream_t s);
+// out[i + nu j] = c u[i] v[j].
+void outer(const double* u, int64_t nu, const double* v, int64_t nv, double c, double* out, hipStream_t s);
 
 }  // namespace uw12g

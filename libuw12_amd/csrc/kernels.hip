@@ -534,6 +534,63 @@ void pair_gather(const double* in, int64_t rows, int a, int c, int j0, int nj, i
   UW12G_HIP(hipGetLastError());
 }
 
+// Euclidean norm of each row of a column-major rows x nt matrix with column stride `stride`.
+__global__ void row_norms_kernel(const double* x, int64_t rows, int64_t stride, int nt, double* out) {
+  const int64_t r = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (r >= rows) return;
+  double acc = 0.0;
+  for (int t = 0; t < nt; ++t) {
+    const double v = x[r + stride * t];
+    acc += v * v;
+  }
+  out[r] = sqrt(acc);
+}
+
+// Maxima of a column-major n0 x n1 matrix: over its columns (out[i], i < n0) or over its rows (out[j], j < n1).
+__global__ void max_rows_kernel(const double* x, int64_t n0, int64_t n1, double* out) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n0) return;
+  double m = 0.0;
+  for (int64_t j = 0; j < n1; ++j) m = fmax(m, x[i + n0 * j]);
+  out[i] = m;
+}
+__global__ void max_cols_kernel(const double* x, int64_t n0, int64_t n1, double* out) {
+  const int64_t j = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (j >= n1) return;
+  double m = 0.0;
+  for (int64_t i = 0; i < n0; ++i) m = fmax(m, x[i + n0 * j]);
+  out[j] = m;
+}
+
+__global__ void outer_kernel(const double* u, int64_t nu, const double* v, int64_t nv, double c, double* out) {
+  const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (t >= nu * nv) return;
+  out[t] = c * u[t % nu] * v[t / nu];
+}
+
+void row_norms(const double* x, int64_t rows, int64_t stride, int nt, double* out, hipStream_t s) {
+  if (rows <= 0) return;
+  hipLaunchKernelGGL(row_norms_kernel, dim3(unsigned((rows + 255) / 256)), dim3(256), 0, s, x, rows, stride, nt, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void max_over(const double* x, int64_t n0, int64_t n1, bool over_cols, double* out, hipStream_t s) {
+  const int64_t n = over_cols ? n0 : n1;
+  if (n <= 0) return;
+  if (over_cols)
+    hipLaunchKernelGGL(max_rows_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, n0, n1, out);
+  else
+    hipLaunchKernelGGL(max_cols_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, n0, n1, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void outer(const double* u, int64_t nu, const double* v, int64_t nv, double c, double* out, hipStream_t s) {
+  const int64_t n = nu * nv;
+  if (n <= 0) return;
+  hipLaunchKernelGGL(outer_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, u, nu, v, nv, c, out);
+  UW12G_HIP(hipGetLastError());
+}
+
 void fill(double* p, int64_t n, double v, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);

@@ -56,6 +56,7 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int naive) {
   try {
+    if (naive < 0 || naive > 5) throw Error(-1, "mode must be 0 .. 5");
     UW12G_HIP(hipSetDevice(device));
     Dev a, b, c, k;
     a.up(A, nA);
@@ -87,7 +88,7 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
     if (naive == 1) {
       gemm_f64_naive(d, nullptr);
     } else {
-      d.prec = naive >= 2 ? naive - 1 : 0;  // 2: FP32, 3: BF16x3, 4: BF16x6
+      d.prec = naive >= 2 ? naive - 1 : 0;  // 2: FP32, 3: BF16x3, 4: BF16x6, 5: FP16x3
       gemm_f64(d, nullptr, &ws);
     }
     UW12G_HIP(hipDeviceSynchronize());
@@ -108,7 +109,8 @@ extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int prec, int reps, double* ms) {
   try {
-    if (prec < 0 || prec > 3) throw Error(-1, "prec must be 0 (FP64), 1 (FP32), 2 (BF16x3) or 3 (BF16x6)");
+    if (prec < 0 || prec > 4)
+      throw Error(-1, "prec must be 0 (FP64), 1 (FP32), 2 (BF16x3), 3 (BF16x6) or 4 (FP16x3)");
     if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw Error(-1, "M, N, K and batch must be positive");
     // every element the descriptor addresses lies inside the caller's arrays (the
     // maps are checked non-negative, so the largest offset is the sum of the maxima)

@@ -199,8 +199,11 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
   if (mixed && !exact)
     for (const std::string& x : ctx->exact_extra)
       if (x == stage) exact = true;
-  // GEMM arithmetic of the reduced contractions: 1 FP32, 2 / 3 BF16 with two / three planes
-  const int reduced = ctx->prec == UW12G_MIXED_BF16X3 ? 2 : ctx->prec == UW12G_MIXED_BF16X6 ? 3 : 1;
+  // GEMM arithmetic of the reduced contractions: 1 FP32, 2 / 3 BF16 with two / three planes, 4 FP16 planes
+  const int reduced = ctx->prec == UW12G_MIXED_BF16X3   ? 2
+                      : ctx->prec == UW12G_MIXED_BF16X6 ? 3
+                      : ctx->prec == UW12G_MIXED_FP16X3 ? 4
+                                                        : 1;
   e.prec = mixed ? (exact ? 0 : reduced) : ctx->prec;
   gemm_f64(e, ctx->stream, ctx->ws_active, &ctx->prof);
 }
@@ -1233,6 +1236,17 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         // index runs over the Mr rotated rows m throughout.
         double* R = ctx->buf("blk4", size_t(DW) * N * nj);
         double* Lc = ctx->buf("blk5", size_t(Mr) * a * std::min<int64_t>(DW, nA_max));
+        // FP16 split form: the row scales of K~ / S (L's rows (m, beta), R's columns (alpha, j))
+        // and of K~'s factors come from Cauchy-Schwarz bounds, |K~[m, beta, k | j]| <=
+        // ||PhiT_Y[m, j, :]|| ||Phi_Y[beta, k, :]||, instead of passes over those tensors
+        const bool cs = ctx->prec == UW12G_MIXED_FP16X3;
+        double* nrmT = cs ? ctx->buf("cs_nrmT", size_t(Mr) * a) : nullptr;  // ||PhiT_Y[m, j, :]||
+        double* nrmP = cs ? ctx->buf("cs_nrmP", size_t(N) * o) : nullptr;   // ||Phi_Y[beta, k, :]||
+        double* csu = cs ? ctx->buf("cs_u", size_t(Mr)) : nullptr;          // max over j
+        double* csw = cs ? ctx->buf("cs_w", size_t(a)) : nullptr;           // max over m
+        double* csv = cs ? ctx->buf("cs_v", size_t(N)) : nullptr;           // max over k
+        double* hintL = cs ? ctx->buf("cs_hintL", MrN) : nullptr;
+        double* hintR = cs ? ctx->buf("cs_hintR", size_t(N) * nj) : nullptr;
         // fock_i (X = W, Y = V) then fock_j (X = V, Y = W)
         for (int pass = 0; pass < 2; ++pass) {
           const uw12g_base_integrals* X = pass == 0 ? W : V;
@@ -1240,6 +1254,15 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           const double* phitY = pass == 0 ? phitVs : phitWs;
           const double* phiY = pass == 0 ? q.phiV : q.phiW;
           const int DX = X->D, DY = Y->D;
+          if (cs) {
+            row_norms(phitY, int64_t(Mr) * a, int64_t(Mr) * a, DY, nrmT, st);
+            row_norms(phiY, int64_t(N) * o, int64_t(N) * o, DY, nrmP, st);
+            max_over(nrmT, Mr, a, true, csu, st);
+            max_over(nrmT, Mr, a, false, csw, st);
+            max_over(nrmP, N, o, true, csv, st);
+            // L's rows: K~ (first pass) or the pair sums S of two K~ elements (second pass)
+            outer(csu, Mr, csv, N, pass == 1 ? 2.0 : 1.0, hintL, st);
+          }
           timed(ctx, "3e_ind_fock_x2t", "transpose_scale_kernel", 0.0, 16.0 * ao * DX,
                 [&] { transpose_scale(pass == 0 ? q.o2W : q.o2V, ao, DX, X->sign.p, X2sT, st); });
           const bool paired = pass == 1;
@@ -1269,15 +1292,18 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 g.Cm = two(uint32_t(Mr), 1, int64_t(MrN) * pb.nk);
                 g.Cn = lin(Mr);
                 g.M = Mr * pb.nb;
+                if (cs) g.hintA = nrmT + size_t(Mr) * pb.j0;
                 if (c > 0) {
                   g.N = N * c;
                   g.B = phiY;
                   g.C = S;
+                  if (cs) g.hintB = nrmP;
                   run_gemm(ctx, g, "3e_ind_fock_kt");
                 }
                 g.N = N * (a - pb.j0);
                 g.B = phiY + size_t(N) * (c + pb.j0);
                 g.C = S + MrN * size_t(c);
+                if (cs) g.hintB = nrmP + size_t(N) * (c + pb.j0);
                 run_gemm(ctx, g, "3e_ind_fock_kt");
                 if (nbeyond > 0) {
                   // S[., c + k - j0 | j] += K~[m, (beta, c+j) | k] for the active k past the block:
@@ -1296,6 +1322,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                   h.C = S + MrN * size_t(c + pb.nb);
                   h.Cm = two(uint32_t(Mr), 1, int64_t(MrN));
                   h.Cn = two(uint32_t(N), Mr, int64_t(MrN) * pb.nk);
+                  if (cs) {
+                    h.hintA = nrmT + size_t(Mr) * (pb.j0 + pb.nb);
+                    h.hintB = nrmP + size_t(N) * (c + pb.j0);
+                  }
                   run_gemm(ctx, h, "3e_ind_fock_kt_pair");
                 }
               }
@@ -1314,6 +1344,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.C = L;
               h.Cm = lin(1);
               h.Cn = lin(int64_t(MrN));
+              h.hintA = hintL;
               run_gemm(ctx, h, "3e_ind_fock_l");
               progress(ctx, "same-spin 3e Fock, spin %d pass %d, A %d-%d of %d, pair group %d of %d (j %d-%d)", s,
                        pass, A0, A0 + An, DX, int(gi) + 1, int(pgrp.size()), pblk[pgrp[gi].first].j0,
@@ -1336,6 +1367,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               g.C = Kt;
               g.Cm = two(uint32_t(Mr), 1, MNo);
               g.Cn = lin(Mr);
+              if (cs) {
+                g.hintA = nrmT + size_t(Mr) * j0;
+                g.hintB = nrmP;
+              }
               run_gemm(ctx, g, "3e_ind_fock_kt");
               GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
               h.M = int(MrN);
@@ -1351,6 +1386,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.C = L;
               h.Cm = lin(1);
               h.Cn = lin(int64_t(MrN));
+              h.hintA = hintL;
               run_gemm(ctx, h, "3e_ind_fock_l");
               if (pass == 0 && A0 == 0) {
                 // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j], with jf
@@ -1378,6 +1414,12 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 r.Cm = lin(1);
                 r.Cn = lin(DW);
                 r.Cb = int64_t(DW) * N * jf;
+                if (cs) {
+                  // column (alpha, j) of the block: |K~[m, alpha, c+i | j]| <= w_j v_alpha
+                  outer(csv, N, csw + j0, jn, 1.0, hintR, st);
+                  r.hintB = hintR;
+                  r.hBb = int64_t(N) * jf;
+                }
                 run_gemm(ctx, r, "3e_ind_fock_r");
                 GemmDesc f;  // F += k1 sum_{(A, j)} sW_A Phi_W[beta, c+j, A] R[A, alpha | j]
                 f.M = N;
@@ -1668,7 +1710,7 @@ int uw12g_form_fock(uw12g_context* ctx, const uw12g_base_integrals* W, const uw1
                     int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                     double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_FP16X3, "unsupported precision", -2);
     require(ctx != nullptr, "null context");
     double t6[6] = {0, 0, 0, 0, 0, 0};
     double* tp = terms_out ? terms_out : t6;
@@ -1712,7 +1754,7 @@ int uw12g_form_fock_noalloc(uw12g_context* ctx, const uw12g_base_integrals* W, c
                             const size_t* n_active, int indirect_term, double scale_opp_spin, double scale_same_spin,
                             uw12g_precision precision, double* fock, double* terms_out, double* energy) {
   return guard([&] {
-    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_FP16X3, "unsupported precision", -2);
     require(ctx != nullptr, "null context");
     double* base = scratch_address(ctx->device);
     require(base != nullptr, "no scratch array on this device", -3);
@@ -1732,7 +1774,7 @@ int uw12g_forward(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g
                   int indirect_term, double scale_opp_spin, double scale_same_spin, uw12g_precision precision,
                   double* terms_out, double* energy, unsigned long long* tape) {
   return guard([&] {
-    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_BF16X6, "unsupported precision", -2);
+    require(precision >= UW12G_FP64 && precision <= UW12G_MIXED_FP16X3, "unsupported precision", -2);
     require(ctx != nullptr && tape != nullptr, "null argument");
     require(C != nullptr && occ != nullptr && n_occ != nullptr && n_active != nullptr, "null input array");
     require(n_spin == 1 || n_spin == 2, "Number of spin channels must be 1 or 2");

@@ -87,8 +87,16 @@ struct GemmDesc {
   Map2 Km;
   int64_t Kb = 0;
   // matrix-core arithmetic: 0 FP64, 1 FP32 (operands and chunk sums in FP32, FP64 storage),
-  // 2 / 3 BF16 split products with two / three planes per operand (FP32 chunk sums)
+  // 2 / 3 BF16 split products with two / three planes per operand, 4 FP16 split products
+  // with two half planes of row-scaled operands (FP32 chunk sums)
   int prec = 0;
+  // FP16 split form only (optional): upper bounds on max_k |A(m, k) ks(k)| at hintA[b hAb + m]
+  // and on max_k |B(k, n)| at hintB[b hBb + n]; the row scales then come from them instead of
+  // a pass over the operand
+  const double* hintA = nullptr;
+  int64_t hAb = 0;
+  const double* hintB = nullptr;
+  int64_t hBb = 0;
 };
 
 // Allocation registry: every device buffer the library allocates is recorded

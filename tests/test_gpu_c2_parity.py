@@ -10,7 +10,8 @@ Bars (absolute):
   1e-10 (tests/test_oracle.py, tests/test_gpu_parity.py);
 * well-conditioned random inputs of the same shape: 1e-12 relative per term,
   1e-10 of max|F|;
-* the mixed form against the FP64 GPU path on the recipe: 1e-6 Eh, 1e-6 Fock.
+* the mixed form and its FP16x3 split variant against the FP64 GPU path on
+  the recipe: 1e-6 Eh, 1e-6 Fock.
 The inputs are regenerated and checked against the fixture (SHA-256 digests for
 the recipe, the arrays' sums for the random case)."""
 import os
@@ -60,11 +61,13 @@ def test_c2_recipe_vs_oracle(gpu_ctx, fixture, cases, name):
         assert abs(eg[key] - eo[key]) <= bar, (key, eg[key], eo[key])
     assert fock_error(Fg, [Fo]) <= 3e-8
     W, V, WV, P = h
-    em, Fm = u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
-                         scale_opp_spin=kappa[0], scale_same_spin=kappa[1], precision=u.MIXED)
-    for key in KEYS:
-        assert abs(em[key] - eg[key]) <= 1e-6, (key, em[key], eg[key])
-    assert fock_error(Fm, Fg) <= 1e-6
+    for form in (u.MIXED, u.MIXED_FP16X3):
+        em, Fm = u.form_fock(W, V, WV, P, system['orbitals'], system['occ'], system['n_active'],
+                             scale_opp_spin=kappa[0], scale_same_spin=kappa[1], precision=form)
+        print(f'  form {form}: Fock {fock_error(Fm, Fg):.2e}', flush=True)
+        for key in KEYS:
+            assert abs(em[key] - eg[key]) <= 1e-6, (form, key, em[key], eg[key])
+        assert fock_error(Fm, Fg) <= 1e-6
 
 
 def test_c2_random_vs_oracle(gpu_ctx, fixture, cases):

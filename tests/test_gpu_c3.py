@@ -53,7 +53,8 @@ def test_c3_precision_forms(c3):
           flush=True)
     assert 0.0 < dm <= 1e-6
     assert d32 <= 1e-4
-    for form, name, bar in ((u.MIXED_BF16X6, 'BF16x6', 1e-6), (u.MIXED_BF16X3, 'BF16x3', 1e-4)):
+    for form, name, bar in ((u.MIXED_BF16X6, 'BF16x6', 1e-6), (u.MIXED_BF16X3, 'BF16x3', 1e-4),
+                            (u.MIXED_FP16X3, 'FP16x3', 1e-6)):
         print(f'case c3 {name}', flush=True)
         eb, Fb = u.form_fock(*args, **kw, precision=form)
         for k in TERMS + ('total',):

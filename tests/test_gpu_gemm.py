@@ -1,4 +1,4 @@
-"""The strided MFMA GEMM kernel (FP64, FP32 and BF16 split-product forms) against the plain kernel and numpy."""
+"""The strided MFMA GEMM kernel (FP64, FP32, BF16 and FP16 split-product forms) against the plain kernel and numpy."""
 import numpy as np
 import pytest
 
@@ -118,15 +118,21 @@ def fp32_bound(M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, 
 # and the dropped ll is below 2^-16: 3 * 2^-16 in all.  Mode 4 (three planes, six
 # products hh hm mh hl mm lh): operands to about 2^-24, dropped ml + lm + ll below
 # 2^-23: 2^-22 in all.  Small factors cover the FP32 roundings inside the splits.
-BF16_FORMS = {3: (3.1 * 2.0 ** -16, 3), 4: (1.1 * 2.0 ** -22, 6)}
+# Mode 5 (FP16x3: rows scaled by powers of two, two half planes, hh + hl + lh):
+# each operand is h + l to 2^-22 (11 significant bits per plane) and the dropped
+# ll is below 2^-22: 3 * 2^-22; the in-kernel fallback (no workspace) runs the
+# three-plane BF16 form, so the FP32 terms are counted as six per k.
+BF16_FORMS = {3: (3.1 * 2.0 ** -16, 3), 4: (1.1 * 2.0 ** -22, 6), 5: (3.1 * 2.0 ** -22, 6)}
 
 
 def reduced_bound(mode, M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, ks=None, Km=None,
                   chunk=4096):
     """Elementwise bound on |C - C_exact| for a reduced form: mode 2 the FP32 form
-    (fp32_bound), modes 3 / 4 the BF16 split forms: (per-product error + gamma_n)
+    (fp32_bound), modes 3 / 4 / 5 the split forms: (per-product error + gamma_n)
     times |A| |B|, with n the FP32 additions of one k chunk (products per k times
-    the chunk length)."""
+    the chunk length).  Mode 5 adds the half planes' subnormal floor: elements of
+    a row scaled to a maximum in [2^14, 2^15) keep an absolute error below 2^-25
+    in l, i.e. 2^-38 of the row maximum, bounded here by 2^-37 K max|A| max|B|."""
     if mode == 2:
         return fp32_bound(M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm, Cn, Cb, ks, Km, chunk)
     rel, per_k = BF16_FORMS[mode]
@@ -136,7 +142,11 @@ def reduced_bound(mode, M, N, K, batch, alpha, A, Am, Ak, Ab, B, Bk, Bn, Bb, Cm,
     absC = host_gemm(M, N, K, batch, abs(alpha), 0.0, np.abs(A), Am, Ak, Ab, np.abs(B), Bk, Bn, Bb,
                      np.zeros(_c_size(M, N, batch, Cm, Cn, Cb)), Cm, Cn, Cb,
                      None if ks is None else np.abs(ks), Km)
-    return (rel + g) * absC
+    floor = 0.0
+    if mode == 5:
+        ka = 1.0 if ks is None else np.abs(ks).max()
+        floor = 2.0 ** -37 * K * abs(alpha) * np.abs(A).max() * ka * np.abs(B).max()
+    return (rel + g) * absC + floor
 
 
 def _c_size(M, N, batch, Cm, Cn, Cb):
@@ -187,12 +197,13 @@ def bf16_path(request):
 
 @pytest.mark.parametrize('name,M,N,K,batch', CASES)
 @pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
-@pytest.mark.parametrize('mode', [3, 4])
+@pytest.mark.parametrize('mode', [3, 4, 5])
 @pytest.mark.parametrize('bf16_path', ['planes', 'inkernel', 'pieces'], indirect=True)
 def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode, bf16_path):
-    """BF16 split forms (two and three planes) within their error bound, with the
-    operands split into plane tensors, inside the kernel, or in pieces; the
-    three-plane form is within a small factor of the FP32 form's error."""
+    """Split forms (two and three bfloat16 planes, two half planes) within their
+    error bound, with the operands split into plane tensors, inside the kernel,
+    or in pieces; the three-plane BF16 and the FP16 forms are within a small
+    factor of the FP32 form's error."""
     rng = np.random.default_rng(hash(('bf16', name, layout, mode)) % 2**32)
     A = rng.standard_normal(M * K * batch)
     B = rng.standard_normal(K * N * batch)
@@ -207,12 +218,41 @@ def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode, bf
     bound = bound + 1e-15 * np.abs(ref)  # the beta C term, in FP64
     assert np.all(err <= bound), (err / bound).max()
     assert err.max() > 0
-    if mode == 4 and K >= 16:
+    if mode in (4, 5) and K >= 16:
         e32 = np.abs(run(*args, naive=2) - ref).max()
         assert err.max() <= 8 * e32, (err.max(), e32)
 
 
-@pytest.mark.parametrize('mode', [0, 2, 3, 4])
+@pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
+def test_fp16_form_rows_of_wide_range(layout):
+    """FP16x3 with operand rows and columns spread over 2^-40 .. 2^40 (beyond the
+    half range of 2^-24 .. 65504 for any single scale): every row of A and column
+    of B gets its own power-of-two scale, so each output element stays within
+    the split forms' bound of its own |A| |B| and the error is far below the
+    two-plane BF16 form's on the same operands."""
+    M, N, K = 200, 136, 300
+    rng = np.random.default_rng(hash(('wide', layout)) % 2**32)
+    A = rng.standard_normal((M, K)) * np.exp2(rng.uniform(-40, 40, M))[:, None]
+    B = rng.standard_normal((K, N)) * np.exp2(rng.uniform(-40, 40, N))[None, :]
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    Af = A.flatten(order='F') if layout[0] == 'N' else A.flatten(order='C')
+    Bf = B.flatten(order='F') if layout[1] == 'N' else B.flatten(order='C')
+    args = (M, N, K, 1, 1.0, 0.0, Af, Am, Ak, 0, Bf, Bk, Bn, 0, np.zeros(M * N), M2(1), M2(M), 0)
+    ref = host_gemm(*args)
+    got = run(*args, naive=5)
+    absC = (np.abs(A) @ np.abs(B)).flatten(order='F')
+    u32 = 2.0 ** -24
+    g = (6 * K + 2) * u32 / (1 - (6 * K + 2) * u32)
+    rel = np.abs(got - ref) / absC
+    assert np.all(rel <= 3.1 * 2.0 ** -22 + g), rel.max()
+    rel3 = np.abs(run(*args, naive=3) - ref) / absC
+    # the FP32 sums bound both forms' largest relative errors from below: measured 1.4e-7 (FP16x3)
+    # against 1.5e-6 (BF16x3)
+    assert rel.max() <= rel3.max() / 4, (rel.max(), rel3.max())
+
+
+@pytest.mark.parametrize('mode', [0, 2, 3, 4, 5])
 @pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
 @pytest.mark.parametrize('kc', [False, True])
 @pytest.mark.parametrize('n0', [32, 16])
@@ -253,7 +293,7 @@ def test_affine_two_level_k_maps(mode, K, kc, n0, bf16_path):
 
 
 @pytest.mark.parametrize('side', ['A', 'B'])
-@pytest.mark.parametrize('mode', [0, 2, 3, 4])
+@pytest.mark.parametrize('mode', [0, 2, 3, 4, 5])
 @pytest.mark.parametrize('bf16_path', ['planes', 'inkernel'], indirect=True)
 def test_operand_span_past_lane_offsets(side, mode, bf16_path):
     """An operand whose row (A) or column (B) offsets pass 2^28 doubles, the limit of the

@@ -256,15 +256,18 @@ def test_bf16_split_forms_errors(gpu_ctx, name, kappa):
     rounding of the operands): the accuracy gate, 1e-6 Eh per term and 1e-6
     Fock max-abs.  Two planes (~3 * 2^-16 per product): e3 indirect within 1e-5
     Eh and the Fock within 5e-5 (measured 7e-7 and 5e-6 at n_ao = 320), and no
-    more accurate than the three-plane form."""
-    print(f'case {name} kappa={kappa} BF16 split forms', flush=True)
+    more accurate than the three-plane form.  Two half planes of row-scaled
+    operands (FP16x3, ~3 * 2^-22 per product): the accuracy gate, and more
+    accurate than the two-plane BF16 form."""
+    print(f'case {name} kappa={kappa} BF16 / FP16 split forms', flush=True)
     s = synth.make_system(name)
     W, V, WV, P = gpu_setup(s, gpu_ctx)
     args = (W, V, WV, P, s['orbitals'], s['occ'], s['n_active'])
     kw = dict(indirect_term=True, scale_opp_spin=kappa[0], scale_same_spin=kappa[1])
     e64, F64 = u.form_fock(*args, **kw, precision=u.FP64)
     errs = {}
-    for form, e_bar, f_bar in ((u.MIXED_BF16X6, 1e-6, 1e-6), (u.MIXED_BF16X3, 1e-5, 5e-5)):
+    for form, e_bar, f_bar in ((u.MIXED_BF16X6, 1e-6, 1e-6), (u.MIXED_BF16X3, 1e-5, 5e-5),
+                               (u.MIXED_FP16X3, 1e-6, 1e-6)):
         e, F = u.form_fock(*args, **kw, precision=form)
         for k in TERMS:
             if k == 'e3_indirect' and kappa[1] != 0.0:
@@ -276,6 +279,7 @@ def test_bf16_split_forms_errors(gpu_ctx, name, kappa):
               flush=True)
         assert 0.0 < errs[form] <= f_bar
     assert errs[u.MIXED_BF16X6] <= errs[u.MIXED_BF16X3]
+    assert errs[u.MIXED_FP16X3] <= errs[u.MIXED_BF16X3]
 
 
 def test_fp32_form_errors(gpu_ctx):
