@@ -5,25 +5,27 @@ One step = one full uw12_fock evaluation (2-, 3- and 4-electron energy terms
 and the Fock matrix) of a synthetic s-Gaussian cluster whose density-fitted
 integrals are already resident in HBM (set up once, as an SCF caller would).
 
-The timed form is the mixed one by default (FP32 matrix cores for the large
-intermediate tensors of the Fock terms, FP64 for the orbital transforms, the
-energies and every accumulation into the Fock matrix).  The plain FP64 path is
-timed beside it on the same inputs, and the mixed result is gated against it:
-`accuracy_gate` reports the largest energy-term and Fock deviations and
-whether both are within 1e-6.  `--form fp64` times the plain path instead;
-`--form bf16x3` / `bf16x6` / `fp16x3` time the mixed form with its Fock
-intermediates as BF16 or FP16 split products.  The side forms (FP32, mixed,
-BF16x3, BF16x6, FP16x3) are timed
-and compared with FP64 after the timed steps, and so is the differentiation
-form (forward then adjoint, `adjoint_form`).
+The timed form is by default the mixed form with FP16x3 split products
+(`--form fp16x3`): the large intermediate tensors of the Fock terms run on the
+FP16 matrix cores as three products of two IEEE half planes per operand (rows
+scaled by powers of two, FP32 sums), while the orbital transforms, the
+energies and every accumulation into the Fock matrix stay FP64.  The plain
+FP64 path is timed beside it on the same inputs, and the timed result is gated
+against it: `accuracy_gate` reports the largest energy-term and Fock
+deviations and whether both are within 1e-6.  `--form fp64` times the plain
+path instead; `--form mixed` the mixed form with those intermediates on the
+FP32 matrix cores; `--form bf16x3` / `bf16x6` with BF16 split products.  The
+side forms (FP32, mixed, BF16x3, BF16x6, FP16x3) are timed and compared with
+FP64 after the timed steps, and so is the differentiation form (forward then
+adjoint, `adjoint_form`).
 
 Workloads (--config): c1, c2 (host integral arrays, the default timed
 workload c2), c3 / c4 (integrals generated on the device and stored), c5
 (n_ao = 1500: integrals generated on the device and not stored -- every
 evaluation regenerates its three-index blocks).  One c5 evaluation takes about
 half a minute, so the default run times c2 for the requested steps and then
-measures c5 at kappa = (1, 0) once per form (`c5_form`: mixed and FP64, the
-accuracy gate, its own roofline); `--c5-side skip` leaves that out.  Before
+measures c5 at kappa = (1, 0) once per form (`c5_form`: the timed form and
+FP64, the accuracy gate, its own roofline); `--c5-side skip` leaves that out.  Before
 that, `c3_forms` (`--c3-side skip` leaves it out) gives one evaluation per
 precision form at n_ao = 800 (unrestricted, kappa = (1, 0)): time and per-term
 and Fock deviation from FP64.
@@ -438,7 +440,7 @@ def main():
     ap.add_argument('--kappa', default=None, help='k0,k1 (default: (1,0.5) for c1/c2, (1,0) for c3-c5)')
     ap.add_argument('--integrals', default=None, choices=['host', 'device', 'streamed'],
                     help='default: host for c1/c2, device for c3/c4, streamed for c5')
-    ap.add_argument('--form', default='mixed', choices=['mixed', 'fp64', 'fp32', 'bf16x3', 'bf16x6', 'fp16x3'],
+    ap.add_argument('--form', default='fp16x3', choices=['mixed', 'fp64', 'fp32', 'bf16x3', 'bf16x6', 'fp16x3'],
                     help='precision form of the timed steps (the plain FP64 path is run beside it)')
     ap.add_argument('--side-reps', type=int, default=3,
                     help='timed evaluations of each side form (after one untimed one when more than 1)')
@@ -449,7 +451,7 @@ def main():
     ap.add_argument('--c3-side', default='auto', choices=['auto', 'skip'],
                     help='auto: after the timed c2 steps, one c3 evaluation per precision form')
     ap.add_argument('--c5-side', default='auto', choices=['auto', 'skip'],
-                    help='auto: after the timed c2 steps, one c5 evaluation per form (mixed timed, FP64 gate)')
+                    help='auto: after the timed c2 steps, one c5 evaluation per form (the timed form, FP64 gate)')
     ap.add_argument('--cpu-baseline', default='auto', choices=['auto', 'skip', 'full'],
                     help='auto: sampled oracle timing; full: one complete oracle evaluation, with parity report')
     ap.add_argument('--profile-table', action='store_true', help='also print the per-kernel table to stderr')
@@ -543,11 +545,12 @@ def main():
             ctx.set_verbosity(1)
         k5, i5 = CONFIG_DEFAULTS['c5']
         sys5, h5, t5 = setup_config('c5', i5, ctx)
-        m5 = measure('c5', sys5, h5, ctx, k5, 'mixed', 1, 1, 'gate', 1, energy_only=False, label='c5')
+        f5 = 'mixed' if args.form in ('fp64', 'fp32') else args.form  # streamed integrals: no FP32 form
+        m5 = measure('c5', sys5, h5, ctx, k5, f5, 1, 1, 'gate', 1, energy_only=False, label='c5')
         r5 = m5['roofline']
-        r5['traffic'] = measured_traffic(r5['kernel'], 'c5', 'mixed')
+        r5['traffic'] = measured_traffic(r5['kernel'], 'c5', f5)
         out['c5_form'] = {
-            'workload': workload('c5', sys5, k5, 'mixed', i5),
+            'workload': workload('c5', sys5, k5, f5, i5), 'form': f5,
             'evals_per_s': round(m5['value'], 6), 'ms_per_eval': round(1e3 * m5['elapsed'], 3),
             'steps': 1, 'warmup': 1, 'fp64_form': m5['fp64'], 'accuracy_gate': m5['gate'], 'roofline': r5,
             'energy': m5['terms']['total'], 'executed_flops_per_eval': m5['gpu_flops'],

@@ -1118,12 +1118,15 @@ struct PlaneArgs {
 // layout, the same MFMA plane products.  F16: the planes are IEEE halves
 // (v_mfma_f32_16x16x32_f16) and the epilogue undoes the row scales.
 template <int BM, int BN, int NP, bool F16>
-__global__ __launch_bounds__(NT) void gemm_planes_kernel(Params p, PlaneArgs pa) {
+__global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs pa) {
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int UA = 4 * BM / NT, UB = 4 * BN / NT;  // units per thread, plane and slab
-  __shared__ __attribute__((aligned(16))) uint32_t As[NP][BM][kRowWords];
-  __shared__ __attribute__((aligned(16))) uint32_t Bs[NP][BN][kRowWords];
+  // two planes: two LDS buffers (80 KiB at 128 x 128, two workgroups per CU);
+  // three planes: one (its registers allow one set of staged loads)
+  constexpr int NBUF = NP == 2 ? 2 : 1;
+  __shared__ __attribute__((aligned(16))) uint32_t As[NBUF][NP][BM][kRowWords];
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[NBUF][NP][BN][kRowWords];
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -1180,18 +1183,18 @@ __global__ __launch_bounds__(NT) void gemm_planes_kernel(Params p, PlaneArgs pa)
       }
     }
   };
-  auto wr = [&](const u32x4(&ra)[NP][UA], const u32x4(&rb)[NP][UB]) {
+  auto wr = [&](const u32x4(&ra)[NP][UA], const u32x4(&rb)[NP][UB], auto buf) {
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) {
 #pragma unroll
       for (int i = 0; i < UA; ++i) {
         const int u = t + NT * i;
-        *reinterpret_cast<u32x4*>(&As[pl][u % BM][4 * (u / BM)]) = ra[pl][i];
+        *reinterpret_cast<u32x4*>(&As[buf()][pl][u % BM][4 * (u / BM)]) = ra[pl][i];
       }
 #pragma unroll
       for (int i = 0; i < UB; ++i) {
         const int u = t + NT * i;
-        *reinterpret_cast<u32x4*>(&Bs[pl][u % BN][4 * (u / BN)]) = rb[pl][i];
+        *reinterpret_cast<u32x4*>(&Bs[buf()][pl][u % BN][4 * (u / BN)]) = rb[pl][i];
       }
     }
   };
@@ -1203,76 +1206,122 @@ __global__ __launch_bounds__(NT) void gemm_planes_kernel(Params p, PlaneArgs pa)
     for (int j = 0; j < NB; ++j) acc[i][j] = f4{0, 0, 0, 0};
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
-  auto compute = [&]() {
-    u32x4 bfr[NB][NP];
+  // plane products (bfloat16 or half operands), smallest first: 3 products with
+  // two planes, 6 with three; product p multiplies B plane kB[p] by A plane kA[p]
+  constexpr int NPROD = NP == 3 ? 6 : 3;
+  auto mma = [](const u32x4& b, const u32x4& a, f4 c) {
+    if constexpr (F16)
+      return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, b), __builtin_bit_cast(h8, a), c, 0, 0, 0);
+    else
+      return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, b), __builtin_bit_cast(bf8, a), c, 0, 0, 0);
+  };
+  // One slab from LDS buffer `buf`: all fragments read first (those of the first
+  // product first), then the products outermost, so that consecutive MFMAs go to
+  // different accumulators (16 apart for 4 x 4 fragments) and never wait on each
+  // other; each accumulator still sums its products in the same order.
+  auto compute = [&](auto buf) {
+    constexpr int kB[6] = {NP == 3 ? 2 : 1, 0, 1, 1, 0, 0};
+    constexpr int kA[6] = {0, NP == 3 ? 2 : 1, 1, 0, 1, 0};
+    constexpr int first = NP == 3 ? 0 : 3;  // two planes: the last three entries
+    u32x4 afr[MB][NP], bfr[NB][NP];
+#pragma unroll
+    for (int i = 0; i < MB; ++i)
+      afr[i][kA[first]] = *reinterpret_cast<const u32x4*>(&As[buf()][kA[first]][wm0 + 16 * i + lr][4 * lk]);
 #pragma unroll
     for (int j = 0; j < NB; ++j)
+      bfr[j][kB[first]] = *reinterpret_cast<const u32x4*>(&Bs[buf()][kB[first]][wn0 + 16 * j + lr][4 * lk]);
 #pragma unroll
-      for (int pl = 0; pl < NP; ++pl) bfr[j][pl] = *reinterpret_cast<const u32x4*>(&Bs[pl][wn0 + 16 * j + lr][4 * lk]);
-    // one plane product c += b a (bfloat16 or half operands)
-    auto mma = [](const u32x4& b, const u32x4& a, f4 c) {
-      if constexpr (F16)
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(h8, b), __builtin_bit_cast(h8, a), c, 0, 0, 0);
-      else
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf8, b), __builtin_bit_cast(bf8, a), c, 0, 0,
-                                                       0);
-    };
+    for (int pl = 0; pl < NP; ++pl) {
+      if (pl != kA[first]) {
 #pragma unroll
-    for (int i = 0; i < MB; ++i) {
-      u32x4 afr[NP];
+        for (int i = 0; i < MB; ++i)
+          afr[i][pl] = *reinterpret_cast<const u32x4*>(&As[buf()][pl][wm0 + 16 * i + lr][4 * lk]);
+      }
+      if (pl != kB[first]) {
 #pragma unroll
-      for (int pl = 0; pl < NP; ++pl) afr[pl] = *reinterpret_cast<const u32x4*>(&As[pl][wm0 + 16 * i + lr][4 * lk]);
-#pragma unroll
-      for (int j = 0; j < NB; ++j) {
-        f4 c = acc[i][j];
-        if constexpr (NP == 3) {
-          c = mma(bfr[j][2], afr[0], c);
-          c = mma(bfr[j][0], afr[2], c);
-          c = mma(bfr[j][1], afr[1], c);
-        }
-        c = mma(bfr[j][1], afr[0], c);
-        c = mma(bfr[j][0], afr[1], c);
-        acc[i][j] = mma(bfr[j][0], afr[0], c);
+        for (int j = 0; j < NB; ++j)
+          bfr[j][pl] = *reinterpret_cast<const u32x4*>(&Bs[buf()][pl][wn0 + 16 * j + lr][4 * lk]);
       }
     }
+#pragma unroll
+    for (int pr = first; pr < 6; ++pr)
+#pragma unroll
+      for (int i = 0; i < MB; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j) acc[i][j] = mma(bfr[j][kB[pr]], afr[i][kA[pr]], acc[i][j]);
   };
+  // Instruction order of one slab of the two-plane loop (compute, then the LDS
+  // writes of the next slab and the loads of the one after it): the fragment
+  // reads of the first product, then each MFMA paired with one of the other
+  // fragment reads, LDS writes and global loads, so that they issue in the
+  // shadow of the MFMAs.
+  constexpr int NRD0 = MB + NB, NRD1 = (MB + NB) * (NP - 1), NMF = MB * NB * NPROD;
+  constexpr int NWR = (UA + UB) * NP, NLD = (UA + UB) * NP;
+  auto schedule = [&]() {
+    __builtin_amdgcn_sched_group_barrier(0x100, NRD0, 0);
+#pragma unroll
+    for (int i = 0; i < NRD1; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NWR; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x200, 1, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < NLD; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, NMF - NRD1 - NWR - NLD, 0);
+  };
+  static_assert(NP == 3 || NMF >= NRD1 + NWR + NLD, "slab schedule");
 
-  if (nslab > 0 && NP == 3) {
-    // three planes: one register set (two would leave one wave per SIMD);
-    // LDS holds slab s while the loads of slab s + 1 are in flight
-    ld(xa, xb, 0);
-    wr(xa, xb);
-    __syncthreads();
-    ld(xa, xb, 1);
-    for (int s = 0;; ++s) {
-      compute();
-      if (s + 1 >= nslab) break;
+  using B0 = std::integral_constant<int, 0>;
+  if constexpr (NP == 3) {
+    if (nslab > 0) {
+      // three planes: one register set (two would leave one wave per SIMD);
+      // LDS holds slab s while the loads of slab s + 1 are in flight
+      ld(xa, xb, 0);
+      wr(xa, xb, B0{});
       __syncthreads();
-      wr(xa, xb);
-      __syncthreads();
-      ld(xa, xb, s + 2);
+      ld(xa, xb, 1);
+      for (int s = 0;; ++s) {
+        compute(B0{});
+        if (s + 1 >= nslab) break;
+        __syncthreads();
+        wr(xa, xb, B0{});
+        __syncthreads();
+        ld(xa, xb, s + 2);
+      }
     }
   } else if (nslab > 0) {
-    // LDS holds slab s while the registers of one set hold slab s + 1 and the
-    // other the loads of slab s + 2 in flight
+    // slab s is read from LDS buffer s % 2 while the registers of slab s + 1
+    // are written into the other buffer (whose last readers, slab s - 1, all
+    // passed the previous barrier) and then reloaded with slab s + 3; the
+    // other register set holds the loads of slab s + 2 in flight.  One
+    // barrier per slab.  Past the last slab the writes and loads repeat it
+    // (ld clamps the slab), unread.
+    using B1 = std::integral_constant<int, NBUF - 1>;
     ld(xa, xb, 0);
     ld(ya, yb, 1);
-    wr(xa, xb);
+    wr(xa, xb, B0{});
     __syncthreads();
     ld(xa, xb, 2);
     for (int s = 0;; s += 2) {
-      compute();
+      compute(B0{});
+      wr(ya, yb, B1{});
+      ld(ya, yb, s + 3);
+      schedule();
       if (s + 1 >= nslab) break;
       __syncthreads();
-      wr(ya, yb);
-      __syncthreads();
-      ld(ya, yb, s + 3);
-      compute();
+      compute(B1{});
+      wr(xa, xb, B0{});
+      ld(xa, xb, s + 4);
+      schedule();
       if (s + 2 >= nslab) break;
       __syncthreads();
-      wr(xa, xb);
-      __syncthreads();
-      ld(xa, xb, s + 4);
     }
   }
 
