@@ -947,6 +947,10 @@ struct PlaneDesc {
   int R = 0, K = 0, nb = 1, KQ = 0, Rp = 0;
   u32x4* out = nullptr;
   const double* scale = nullptr;  // FP16 form: power-of-two scale of row (b, r) at b Rp + r
+  int x32 = 0;                     // X holds FP32 values
+  __device__ double at(int64_t off) const {
+    return x32 ? double(reinterpret_cast<const float*>(X)[off]) : X[off];
+  }
 };
 
 // One thread per unit, r fastest (the unit writes of a wave are contiguous).
@@ -966,7 +970,7 @@ __global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s) {
     for (int e = 0; e < 8; ++e) {
       const int k = 8 * q + e;
       const bool ok = rok && k < s.K;
-      double x = s.X[ok ? base + s.Xk.off(k) : 0];
+      double x = s.at(ok ? base + s.Xk.off(k) : 0);
       if (s.ks) x *= s.ks[ok ? s.Kb * b + s.Km.off(k) : 0];
       v[e] = ok ? x : 0.0;
     }
@@ -1016,7 +1020,7 @@ __global__ __launch_bounds__(256) void split_planes_kc_kernel(PlaneDesc s) {
       const int r = r0 + row, k = 32 * kt + kk;
       double x = 0.0;
       if (r < s.R && k < s.K) {
-        x = s.X[s.Xb * b + s.Xr.off(r) + s.Xk.off(k)];
+        x = s.at(s.Xb * b + s.Xr.off(r) + s.Xk.off(k));
         if (s.ks) x *= s.ks[s.Kb * b + s.Km.off(k)];
         if constexpr (F16) x *= s.scale[int64_t(b) * s.Rp + r];
       }
@@ -1070,7 +1074,7 @@ __global__ __launch_bounds__(256) void row_max_kernel(PlaneDesc s, int kparts, i
   const int64_t base = s.Xb * b + s.Xr.off(r);
   double m = 0.0;
   for (int k = k0 + lane; k < k1; k += (KC ? 64 : 1)) {
-    double x = s.X[base + s.Xk.off(k)];
+    double x = s.at(base + s.Xk.off(k));
     if (s.ks) x *= s.ks[s.Kb * b + s.Km.off(k)];
     m = fmax(m, fabs(x));
   }
@@ -1354,7 +1358,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
     }
     return;
   }
-  double* Cp = d.C + d.Cb * b;
+  const int64_t cb = d.Cb * b;
 #pragma unroll
   for (int i = 0; i < MB; ++i) {
     const int m = m0 + wm0 + 16 * i + lr;
@@ -1367,9 +1371,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
       for (int r = 0; r < 4; ++r) {
         const int n = ncol(j, r);
         if (n >= d.N) continue;
-        double* c = Cp + mo + d.Cn.off(n);
+        const int64_t off = cb + mo + d.Cn.off(n);
         const double v = colf[j][r] * rf * double(acc[i][j][r]);
-        *c = d.beta == 0.0 ? v : v + d.beta * *c;
+        if (d.c32) {
+          float* c = reinterpret_cast<float*>(d.C) + off;
+          *c = float(d.beta == 0.0 ? v : v + d.beta * double(*c));
+        } else {
+          double* c = d.C + off;
+          *c = d.beta == 0.0 ? v : v + d.beta * *c;
+        }
       }
   }
 }
@@ -1382,8 +1392,14 @@ __global__ void splitk_reduce_kernel(GemmDesc d, const double* W, int splits) {
     double s = 0.0;
     for (int q = 0; q < splits; ++q) s += W[(int64_t(q) * d.batch + b) * MN + e];
     const int m = int(e % d.M), n = int(e / d.M);
-    double* c = d.C + d.Cb * b + d.Cm.off(m) + d.Cn.off(n);
-    *c = d.beta == 0.0 ? s : s + d.beta * *c;
+    const int64_t off = d.Cb * b + d.Cm.off(m) + d.Cn.off(n);
+    if (d.c32) {
+      float* c = reinterpret_cast<float*>(d.C) + off;
+      *c = float(d.beta == 0.0 ? s : s + d.beta * double(*c));
+    } else {
+      double* c = d.C + off;
+      *c = d.beta == 0.0 ? s : s + d.beta * *c;
+    }
   }
 }
 
@@ -1592,11 +1608,11 @@ int64_t map_max(const Map2& m, int64_t extent) {
 
 static void check_desc(const GemmDesc& d) {
   if (d.K > 0) {
-    check_extent(d.A, d.Ab * (d.batch - 1) + map_max(d.Am, d.M) + map_max(d.Ak, d.K), "gemm A");
-    check_extent(d.B, d.Bb * (d.batch - 1) + map_max(d.Bk, d.K) + map_max(d.Bn, d.N), "gemm B");
+    check_extent(d.A, d.Ab * (d.batch - 1) + map_max(d.Am, d.M) + map_max(d.Ak, d.K), "gemm A", d.a32 ? 4 : 8);
+    check_extent(d.B, d.Bb * (d.batch - 1) + map_max(d.Bk, d.K) + map_max(d.Bn, d.N), "gemm B", d.b32 ? 4 : 8);
     if (d.ks) check_extent(d.ks, d.Kb * (d.batch - 1) + map_max(d.Km, d.K), "gemm kscale");
   }
-  check_extent(d.C, d.Cb * (d.batch - 1) + map_max(d.Cm, d.M) + map_max(d.Cn, d.N), "gemm C");
+  check_extent(d.C, d.Cb * (d.batch - 1) + map_max(d.Cm, d.M) + map_max(d.Cn, d.N), "gemm C", d.c32 ? 4 : 8);
 }
 
 namespace {
@@ -1873,9 +1889,9 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
       for (int b0 : {0, h}) {
         GemmDesc e = d;
         e.batch = b0 == 0 ? h : d.batch - h;
-        e.A = d.A + d.Ab * b0;
-        e.B = d.B + d.Bb * b0;
-        e.C = d.C + d.Cb * b0;
+        e.A = advance(d.A, d.Ab * b0, d.a32);
+        e.B = advance(d.B, d.Bb * b0, d.b32);
+        e.C = advance(d.C, d.Cb * b0, d.c32);
         if (d.ks) e.ks = d.ks + d.Kb * b0;
         if (d.hintA) e.hintA = d.hintA + d.hAb * b0;
         if (d.hintB) e.hintB = d.hintB + d.hBb * b0;
@@ -1896,13 +1912,13 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
       const int len = int(std::min<int64_t>(half, ext - x0));
       if (rows) {
         e.M = len;
-        e.A = d.A + d.Am.off(uint32_t(x0));
-        e.C = d.C + d.Cm.off(uint32_t(x0));
+        e.A = advance(d.A, d.Am.off(uint32_t(x0)), d.a32);
+        e.C = advance(d.C, d.Cm.off(uint32_t(x0)), d.c32);
         if (d.hintA) e.hintA = d.hintA + x0;
       } else {
         e.N = len;
-        e.B = d.B + d.Bn.off(uint32_t(x0));
-        e.C = d.C + d.Cn.off(uint32_t(x0));
+        e.B = advance(d.B, d.Bn.off(uint32_t(x0)), d.b32);
+        e.C = advance(d.C, d.Cn.off(uint32_t(x0)), d.c32);
         if (d.hintB) e.hintB = d.hintB + x0;
       }
       if (!gemm_bf16_planes(e, stream, ws, prof)) gemm_f64(e, stream, nullptr, prof);
@@ -1941,6 +1957,7 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   sa.KQ = int(KQ);
   sa.Rp = int(RpA);
   sa.out = const_cast<u32x4*>(pa.A);
+  sa.x32 = d.a32;
   PlaneDesc sb;
   sb.X = d.B;
   sb.Xr = d.Bn;
@@ -1952,6 +1969,7 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   sb.KQ = int(KQ);
   sb.Rp = int(RpB);
   sb.out = const_cast<u32x4*>(pa.B);
+  sb.x32 = d.b32;
   if (f16) {
     if (d.hintA) launch_hint_scale(sa, d.hintA, d.hAb, scA, const_cast<double*>(pa.invA), stream);
     else launch_row_scale(sa, scA, const_cast<double*>(pa.invA), stream, prof);
@@ -2022,6 +2040,9 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   check_desc(d);
   if (d.batch > 65535) throw Error(-2, "gemm batch exceeds 65535");
+  const bool f32s = d.a32 || d.b32 || d.c32;
+  if (f32s && (d.prec < 2 || d.K <= 0))
+    throw Error(-2, "gemm: FP32 operand storage is read only by the split forms over plane tensors");
   if (d.K <= 0) {  // C <- beta C
     GemmDesc z = d;
     z.K = 0;
@@ -2030,6 +2051,7 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
   }
   // BF16 split forms: operands split once into plane tensors when the workspace holds them
   if (d.prec >= 2 && g_bf16_planes && gemm_bf16_planes(d, stream, ws, prof)) return;
+  if (f32s) throw Error(-2, "gemm: FP32 operand storage needs the plane-tensor workspace of the split forms");
   if (d.prec == 4) {
     // the FP16 form needs its row scales and planes in the workspace; without one the
     // product runs as the in-kernel split with three bfloat16 planes (2^-22 per product)

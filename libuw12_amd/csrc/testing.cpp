@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/uw12g_testing.h"
 #include "uw12g_internal.h"
@@ -27,6 +28,19 @@ struct Dev {
     UW12G_HIP(hipMalloc(&p, sizeof(double) * (n ? n : 1)));
     register_alloc(p, sizeof(double) * (n ? n : 1));
     if (h && n) UW12G_HIP(hipMemcpy(p, h, sizeof(double) * n, hipMemcpyHostToDevice));
+  }
+  // FP32 storage: the host values rounded to float, in a buffer of n floats
+  void up32(const double* h, size_t n) {
+    std::vector<float> f(n ? n : 1, 0.0f);
+    for (size_t i = 0; i < n; ++i) f[i] = float(h[i]);
+    UW12G_HIP(hipMalloc(&p, sizeof(float) * f.size()));
+    register_alloc(p, sizeof(float) * f.size());
+    UW12G_HIP(hipMemcpy(p, f.data(), sizeof(float) * f.size(), hipMemcpyHostToDevice));
+  }
+  void down32(double* h, size_t n) const {
+    std::vector<float> f(n ? n : 1);
+    UW12G_HIP(hipMemcpy(f.data(), p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; ++i) h[i] = double(f[i]);
   }
 };
 }  // namespace
@@ -56,12 +70,18 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int naive) {
   try {
-    if (naive < 0 || naive > 5) throw Error(-1, "mode must be 0 .. 5");
+    // mode bits 3..5: A, B, C held in FP32 storage on the device (split forms only)
+    const int store = naive >> 3;
+    naive &= 7;
+    if (naive < 0 || naive > 5) throw Error(-1, "mode must be 0 .. 5 (plus 8 / 16 / 32 for FP32 storage)");
     UW12G_HIP(hipSetDevice(device));
     Dev a, b, c, k;
-    a.up(A, nA);
-    b.up(B, nB);
-    c.up(C, nC);
+    if (store & 1) a.up32(A, nA);
+    else a.up(A, nA);
+    if (store & 2) b.up32(B, nB);
+    else b.up(B, nB);
+    if (store & 4) c.up32(C, nC);
+    else c.up(C, nC);
     if (ks) k.up(ks, nks);
     GemmDesc d;
     d.M = M;
@@ -89,10 +109,14 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
       gemm_f64_naive(d, nullptr);
     } else {
       d.prec = naive >= 2 ? naive - 1 : 0;  // 2: FP32, 3: BF16x3, 4: BF16x6, 5: FP16x3
+      d.a32 = store & 1;
+      d.b32 = (store >> 1) & 1;
+      d.c32 = (store >> 2) & 1;
       gemm_f64(d, nullptr, &ws);
     }
     UW12G_HIP(hipDeviceSynchronize());
-    UW12G_HIP(hipMemcpy(C, c.p, sizeof(double) * nC, hipMemcpyDeviceToHost));
+    if (store & 4) c.down32(C, nC);
+    else UW12G_HIP(hipMemcpy(C, c.p, sizeof(double) * nC, hipMemcpyDeviceToHost));
     return 0;
   } catch (const Error& e) {
     set_last_error(e.what());

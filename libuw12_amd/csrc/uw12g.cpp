@@ -101,7 +101,7 @@ void unregister_alloc(const void* p) {
   allocs().erase(reinterpret_cast<uintptr_t>(p));
 }
 
-void check_extent(const void* base, int64_t max_off, const char* what) {
+void check_extent(const void* base, int64_t max_off, const char* what, int elem) {
   if (!base) throw Error(-2, std::string(what) + ": null operand");
   if (max_off < 0) throw Error(-2, std::string(what) + ": negative extent");
   std::lock_guard<std::mutex> lk(alloc_mu());
@@ -111,7 +111,7 @@ void check_extent(const void* base, int64_t max_off, const char* what) {
   if (it == m.begin()) return;  // not library-owned
   --it;
   if (b >= it->first + it->second) return;  // not inside a library allocation
-  const uintptr_t end = b + uintptr_t(max_off + 1) * sizeof(double);
+  const uintptr_t end = b + uintptr_t(max_off + 1) * uintptr_t(elem);
   if (end > it->first + it->second)
     throw Error(-2, std::string(what) + ": operand extent exceeds its allocation (" +
                         std::to_string(end - it->first) + " > " + std::to_string(it->second) + " bytes)");
@@ -1224,7 +1224,16 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
           np_max = std::max(np_max, np);
           i = e;
         }
-        double* Kt = ctx->buf("blk1", std::max(size_t(MNo) * nj, MrN * size_t(np_max)));
+        // The split forms keep K~ and the pair sums S in FP32 storage: the K~ products sum in
+        // FP32 (over D_Y k), so their values are FP32 numbers up to the combination of k chunks,
+        // and L and R round them again when they split them into planes; this halves the bytes
+        // K~ moves through HBM.  Only the plane-tensor path reads such storage.
+        bool k32 = ctx->prec >= UW12G_MIXED_BF16X3 && g_bf16_planes && !ctx->arena;
+        for (const std::string& x : ctx->exact_extra)
+          if (x.rfind("3e_ind_fock", 0) == 0) k32 = false;
+        const size_t kt_n = std::max(size_t(MNo) * nj, MrN * size_t(np_max));
+        double* Kt = ctx->buf("blk1", k32 ? (kt_n + 1) / 2 : kt_n);
+        auto kt_at = [&](size_t off) { return advance(Kt, int64_t(off), k32); };
         // X2sP[A + DX p] = X2sT[A + DX (j + a k)] for pair p = (j, k) of the packed order
         double* X2sP = ctx->buf("x2sp", size_t(n_pairs) * DV);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
@@ -1279,7 +1288,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               const int64_t pbase = pblk[pgrp[gi].first].p0;
               for (size_t bi = pgrp[gi].first; bi < pgrp[gi].second; ++bi) {
                 const PairBlock& pb = pblk[bi];
-                double* S = Kt + MrN * size_t(pb.p0 - pbase);  // S[(m, beta) + MrN (kk + nk jj)]
+                double* S = kt_at(MrN * size_t(pb.p0 - pbase));  // S[(m, beta) + MrN (kk + nk jj)]
                 const int nbeyond = a - pb.j0 - pb.nb;
                 // S = K~[m, (beta, k) | j], j in the block, k core or active from j0 on; rows (m, j)
                 GemmDesc g;
@@ -1292,6 +1301,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 g.Cm = two(uint32_t(Mr), 1, int64_t(MrN) * pb.nk);
                 g.Cn = lin(Mr);
                 g.M = Mr * pb.nb;
+                g.c32 = k32;
                 if (cs) g.hintA = nrmT + size_t(Mr) * pb.j0;
                 if (c > 0) {
                   g.N = N * c;
@@ -1302,7 +1312,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 }
                 g.N = N * (a - pb.j0);
                 g.B = phiY + size_t(N) * (c + pb.j0);
-                g.C = S + MrN * size_t(c);
+                g.C = advance(S, int64_t(MrN) * c, k32);
                 if (cs) g.hintB = nrmP + size_t(N) * (c + pb.j0);
                 run_gemm(ctx, g, "3e_ind_fock_kt");
                 if (nbeyond > 0) {
@@ -1319,7 +1329,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                   h.B = phiY + size_t(N) * (c + pb.j0);
                   h.Bk = lin(int64_t(N) * o);
                   h.Bn = lin(1);
-                  h.C = S + MrN * size_t(c + pb.nb);
+                  h.C = advance(S, int64_t(MrN) * (c + pb.nb), k32);
+                  h.c32 = k32;
                   h.Cm = two(uint32_t(Mr), 1, int64_t(MrN));
                   h.Cn = two(uint32_t(N), Mr, int64_t(MrN) * pb.nk);
                   if (cs) {
@@ -1336,6 +1347,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.K = int(last.p0 + int64_t(last.nb) * last.nk - pbase);
               h.beta = gi == 0 ? 0.0 : 1.0;
               h.A = Kt;
+              h.a32 = k32;
               h.Am = lin(1);
               h.Ak = lin(int64_t(MrN));
               h.B = X2sP + int64_t(DX) * pbase + A0;
@@ -1365,6 +1377,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               g.Bk = lin(int64_t(N) * o);
               g.Bn = lin(1);
               g.C = Kt;
+              g.c32 = k32;
               g.Cm = two(uint32_t(Mr), 1, MNo);
               g.Cn = lin(Mr);
               if (cs) {
@@ -1378,6 +1391,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.K = o * jn;
               h.beta = j0 == 0 ? 0.0 : 1.0;
               h.A = Kt;
+              h.a32 = k32;
               h.Am = lin(1);
               h.Ak = lin(int64_t(MrN));
               h.B = X2sT + int64_t(DX) * j0 + A0;
@@ -1406,7 +1420,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 r.A = q.psiW + size_t(Mr) * c;
                 r.Am = lin(int64_t(Mr) * o);
                 r.Ak = lin(1);
-                r.B = Kt + MrN * size_t(c);
+                r.B = kt_at(MrN * size_t(c));
+                r.b32 = k32;
                 r.Bk = two(uint32_t(Mr), 1, int64_t(MrN));
                 r.Bn = two(uint32_t(N), Mr, MNo);
                 r.Bb = MNo * jf;

@@ -97,6 +97,9 @@ struct GemmDesc {
   int64_t hAb = 0;
   const double* hintB = nullptr;
   int64_t hBb = 0;
+  // split forms over plane tensors only: A / B / C hold FP32 values (float storage, maps in
+  // elements); the other paths refuse such descriptors
+  int a32 = 0, b32 = 0, c32 = 0;
 };
 
 // Allocation registry: every device buffer the library allocates is recorded
@@ -106,7 +109,14 @@ void register_alloc(const void* p, size_t bytes);
 void unregister_alloc(const void* p);
 // Throws Error(-2) unless [base, base + (max_off + 1) doubles) lies inside one
 // registered allocation; unregistered bases (caller-owned memory) pass.
-void check_extent(const void* base, int64_t max_off, const char* what);
+void check_extent(const void* base, int64_t max_off, const char* what, int elem = 8);
+// Pointer `off` elements of 4 (f32) or 8 bytes past p (FP32 storage behind a double* field).
+inline const double* advance(const double* p, int64_t off, bool f32) {
+  return reinterpret_cast<const double*>(reinterpret_cast<const char*>(p) + off * (f32 ? 4 : 8));
+}
+inline double* advance(double* p, int64_t off, bool f32) {
+  return reinterpret_cast<double*>(reinterpret_cast<char*>(p) + off * (f32 ? 4 : 8));
+}
 // Largest offset a map produces over indices [0, extent).
 int64_t map_max(const Map2& m, int64_t extent);
 

@@ -224,6 +224,44 @@ def test_bf16_forms_within_rounding_bound(name, M, N, K, batch, layout, mode, bf
 
 
 @pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
+@pytest.mark.parametrize('mode', [3, 5])
+@pytest.mark.parametrize('store', [1, 2, 4, 7])
+def test_split_forms_fp32_storage(layout, mode, store):
+    """Split forms reading A / B from FP32 storage and writing C to FP32 storage
+    (test-hook mode bits 8 / 16 / 32; the host values rounded to float): the
+    same error bound as FP64 storage on the rounded operands, plus the final
+    rounding of C to float; beta C read from FP32 storage too."""
+    M, N, K, batch = 300, 200, 150, 2
+    rng = np.random.default_rng(hash(('f32', layout, mode, store)) % 2**32)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    A = rng.standard_normal(M * K * batch)
+    B = rng.standard_normal(K * N * batch)
+    C = rng.standard_normal(M * N * batch)
+    A, B = (f32(A) if store & 1 else A), (f32(B) if store & 2 else B)
+    C = f32(C) if store & 4 else C
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    args = (M, N, K, batch, 1.25, 0.5, A, Am, Ak, M * K, B, Bk, Bn, K * N, C, M2(1), M2(M), M * N)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode + 8 * store)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, Am, Ak, M * K, B, Bk, Bn, K * N, M2(1), M2(M), M * N)
+    bound = bound + (2.0 ** -24 if store & 4 else 1e-15) * np.abs(ref)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound), (err / bound).max()
+    if store & 4:
+        assert np.array_equal(got, f32(got))  # C came back from FP32 storage
+
+
+def test_fp32_storage_needs_the_split_forms():
+    """FP32 operand storage is read only by the plane-tensor path: the FP64 and
+    FP32 forms refuse such descriptors before launching anything."""
+    A, B, C = np.ones(64), np.ones(64), np.zeros(64)
+    for mode in (0, 2):
+        with pytest.raises(_lib.UW12Error):
+            run(8, 8, 8, 1, 1.0, 0.0, A, M2(1), M2(8), 0, B, M2(1), M2(8), 0, C, M2(1), M2(8), 0, naive=mode + 8)
+
+
+@pytest.mark.parametrize('layout', ['NN', 'TN', 'NT', 'TT'])
 def test_fp16_form_rows_of_wide_range(layout):
     """FP16x3 with operand rows and columns spread over 2^-40 .. 2^40 (beyond the
     half range of 2^-24 .. 65504 for any single scale): every row of A and column
