@@ -133,12 +133,15 @@ def roofline_from_profile(records):
 
 
 def traffic_profile(config, form):
-    return f'profiles/gemm_f64_{config}_{form}_traffic.md'
+    """The committed traffic summary of this workload and form (profiles/<kernel>_<config>_<form>_traffic.md)."""
+    import glob
+    hits = sorted(glob.glob(os.path.join(ROOT, 'profiles', f'*_{config}_{form}_traffic.md')))
+    return os.path.relpath(hits[0], ROOT) if hits else f'profiles/gemm_f64_{config}_{form}_traffic.md'
 
 
 def measured_traffic(kernel, config, form):
     """HBM-side bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of
-    this workload (profiles/gemm_f64_<config>_<form>_traffic.md), or None when there is none."""
+    this workload (profiles/<kernel>_<config>_<form>_traffic.md), or None when there is none."""
     path = os.path.join(ROOT, traffic_profile(config, form))
     if not os.path.exists(path):
         return None
