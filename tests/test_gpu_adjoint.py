@@ -19,7 +19,7 @@ def sys_open(gpu_ctx):
     return s, gpu_setup(s, gpu_ctx)
 
 
-@pytest.mark.parametrize('prec', [u.FP64, u.FP32, u.MIXED])
+@pytest.mark.parametrize('prec', [u.FP64, u.FP32, u.MIXED, u.MIXED_FP16X3])
 @pytest.mark.parametrize('kappa', [(1.0, 0.5), (1.0, 0.0)])
 def test_adjoint_equals_fused(sys_open, prec, kappa):
     s, h = sys_open
@@ -34,7 +34,7 @@ def test_adjoint_equals_fused(sys_open, prec, kappa):
     # Fock intermediate L: equal to rounding; every other piece bitwise
     for k in u.TERMS:
         if k == 'e3_indirect':  # reduced forms: the fused path reads it off the FP32 L intermediate
-            bar = {u.FP64: 1e-12, u.FP32: 1e-5, u.MIXED: 1e-6}[prec]
+            bar = {u.FP64: 1e-12, u.FP32: 1e-5, u.MIXED: 1e-6, u.MIXED_FP16X3: 1e-6}[prec]
             assert abs(e[k] - ef[k]) <= bar * max(1.0, abs(ef[k]))
         else:
             assert e[k] == ef[k], k

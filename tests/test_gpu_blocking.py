@@ -20,7 +20,7 @@ def set_cap(ctx, doubles):
 
 
 @pytest.mark.parametrize('name,n_active,cap', [('m6u', None, 20000), ('c1', [4], 2000), ('c1', None, 600)])
-@pytest.mark.parametrize('precision', [u.FP64, u.MIXED])
+@pytest.mark.parametrize('precision', [u.FP64, u.MIXED, u.MIXED_FP16X3])
 def test_blocked_equals_unblocked(gpu_ctx, name, n_active, cap, precision):
     print(f'case {name} n_active={n_active} cap={cap} precision={precision}', flush=True)
     s = synth.make_system(name)
@@ -35,7 +35,8 @@ def test_blocked_equals_unblocked(gpu_ctx, name, n_active, cap, precision):
         q1, _ = u.form_fock(*h, s['orbitals'], s['occ'], act, calculate_fock=False, **kw)
     finally:
         set_cap(gpu_ctx, 0)
-    # FP64: re-association only; mixed: the FP32 k ranges of L follow the j-blocks
+    # FP64: re-association only; mixed forms: the FP32 k ranges of L follow the j-blocks (the FP16x3
+    # form also keeps the blocked K~ in FP32 storage and takes its row scales from bounds)
     rel = 1e-11 if precision == u.FP64 else 1e-6
     scale = max(abs(e0[k]) for k in TERMS)
     for k in TERMS + ('total',):

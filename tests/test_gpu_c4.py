@@ -3,11 +3,12 @@ n_ao=800, n_df=2400, n_ri=3200, closed shell with 160 occupied, kappa = (1, 0),
 integrals generated on the device and stored.
 
 * the adjoint Fock (uw12g_forward + uw12g_adjoint) equals the fused Fock
-  (FP64 to 1e-14 of max|F|; the mixed adjoint within 1e-6 of the FP64 one);
+  (FP64 to 1e-14 of max|F|; the mixed and FP16x3 adjoints within 1e-6 of the
+  FP64 one);
 * the adjoint against Richardson-extrapolated central differences of the FP64
   energy along 4 seeded S-consistent orbital paths C(t) = C + t dC,
   dC = (1 - C C^T S) G, G ~ N(0, 1) 1e-4, t = +-1, +-2 (truncation O(t^4)):
-  FP64 within 1e-9 and mixed within 1e-6 of the largest |dE/dt|."""
+  FP64 within 1e-9, the mixed and FP16x3 forms within 1e-6 of the largest |dE/dt|."""
 import numpy as np
 import pytest
 
@@ -37,7 +38,7 @@ def test_c4_adjoint_and_finite_differences(c4):
     args = (W, V, WV, P)
     kw = dict(indirect_term=False, scale_opp_spin=1.0, scale_same_spin=0.0)
     F_adj = {}
-    for name, prec in (('fp64', u.FP64), ('mixed', u.MIXED)):
+    for name, prec in (('fp64', u.FP64), ('mixed', u.MIXED), ('fp16x3', u.MIXED_FP16X3)):
         print(f'case c4 {name} forward + adjoint', flush=True)
         _, tape = u.forward(*args, orbs, occ, act, **kw, precision=prec)
         F_adj[name] = u.adjoint(tape)
@@ -45,6 +46,7 @@ def test_c4_adjoint_and_finite_differences(c4):
     scale_F = max(np.abs(f).max() for f in F_fused)
     assert max(np.abs(a - b).max() for a, b in zip(F_adj['fp64'], F_fused)) <= 1e-14 * scale_F
     assert max(np.abs(a - b).max() for a, b in zip(F_adj['mixed'], F_fused)) <= 1e-6
+    assert max(np.abs(a - b).max() for a, b in zip(F_adj['fp16x3'], F_fused)) <= 1e-6
     Sao = s['S'][:N, :N]
     rng = np.random.default_rng(4004)
     rows = []
@@ -59,9 +61,10 @@ def test_c4_adjoint_and_finite_differences(c4):
             E[t] = e['total']
         fd = (4.0 * 0.5 * (E[1.0] - E[-1.0]) - 0.25 * (E[2.0] - E[-2.0])) / 3.0
         an = {k: density_derivative(F, orbs, occ, [dC]) for k, F in F_adj.items()}
-        print(f'  direction {d}: fd {fd:.12e}  fp64 {an["fp64"] - fd: .2e}  mixed {an["mixed"] - fd: .2e}',
-              flush=True)
+        print(f'  direction {d}: fd {fd:.12e}  fp64 {an["fp64"] - fd: .2e}  mixed {an["mixed"] - fd: .2e}  '
+              f'fp16x3 {an["fp16x3"] - fd: .2e}', flush=True)
         rows.append((fd, an))
     scale = max(abs(fd) for fd, _ in rows)
     assert max(abs(an['fp64'] - fd) for fd, an in rows) <= 1e-9 * scale
     assert max(abs(an['mixed'] - fd) for fd, an in rows) <= 1e-6 * scale
+    assert max(abs(an['fp16x3'] - fd) for fd, an in rows) <= 1e-6 * scale
