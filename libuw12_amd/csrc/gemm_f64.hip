@@ -62,6 +62,7 @@ struct Params {
   int nfast;    // 1: tiles ordered with the column index fastest
   int group;    // > 1 (with nfast 0): tiles ordered in groups of this many tile rows, row index fastest
   int kbase;    // first k of chunk 0
+  int cvec = 0;  // plane GEMMs: C columns 4 n .. 4 n + 3 contiguous and 4-element aligned (vector epilogue)
 };
 
 // Buffer-resource loads.  The resource base is a wave-uniform address (scalar
@@ -1366,7 +1367,37 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
     const int64_t mo = d.Cm.off(m);
     const double rf = F16 ? invA[m] : 1.0;
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
+    for (int j = 0; j < NB; ++j) {
+      // a lane's four columns contiguous in C: one 16-byte access (FP32) or two (FP64)
+      if (p.cvec && ncol(j, 3) < d.N) {
+        const int64_t off = cb + mo + d.Cn.off(ncol(j, 0));
+        double v[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[r] = colf[j][r] * rf * double(acc[i][j][r]);
+        if (d.c32) {
+          float4* c = reinterpret_cast<float4*>(reinterpret_cast<float*>(d.C) + off);
+          float4 o;
+          if (d.beta == 0.0) {
+            o = float4{float(v[0]), float(v[1]), float(v[2]), float(v[3])};
+          } else {
+            const float4 x = *c;
+            o = float4{float(v[0] + d.beta * double(x.x)), float(v[1] + d.beta * double(x.y)),
+                       float(v[2] + d.beta * double(x.z)), float(v[3] + d.beta * double(x.w))};
+          }
+          *c = o;
+        } else {
+          double2* c = reinterpret_cast<double2*>(d.C + off);
+          double2 o0{v[0], v[1]}, o1{v[2], v[3]};
+          if (d.beta != 0.0) {
+            const double2 x0 = c[0], x1 = c[1];
+            o0 = double2{v[0] + d.beta * x0.x, v[1] + d.beta * x0.y};
+            o1 = double2{v[2] + d.beta * x1.x, v[3] + d.beta * x1.y};
+          }
+          c[0] = o0;
+          c[1] = o1;
+        }
+        continue;
+      }
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int n = ncol(j, r);
@@ -1381,6 +1412,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
           *c = d.beta == 0.0 ? v : v + d.beta * *c;
         }
       }
+    }
   }
 }
 
@@ -1859,6 +1891,15 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   p.tiles_n = (d.N + BN - 1) / BN;
   const int64_t tiles = int64_t(p.tiles_m) * p.tiles_n;
   if (tiles > 0x7fffffff) return false;
+  {
+    // vector epilogue: unit column stride within periods of a multiple of 4, every other C offset
+    // (rows, periods, batches) a multiple of 4 elements, and C 16-byte aligned
+    auto m4 = [](int64_t x) { return x % 4 == 0; };
+    const bool one = d.Cn.n0 == 0xffffffffu;
+    p.cvec = d.Cn.s0 == 1 && (one || (d.Cn.n0 % 4 == 0 && m4(d.Cn.s1))) && m4(d.Cm.s0) &&
+             (d.Cm.n0 == 0xffffffffu || m4(d.Cm.s1)) && m4(d.Cb) &&
+             reinterpret_cast<uintptr_t>(d.C) % 16 == 0 ? 1 : 0;
+  }
   const int kt = int(KQ / 4);
   const int64_t slots = BM == 128 ? (BN == 128 ? planes_slots<128, 128>(kind) : planes_slots<128, 64>(kind))
                                   : (BN == 128 ? planes_slots<64, 128>(kind) : planes_slots<64, 64>(kind));

@@ -208,6 +208,32 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
   gemm_f64(e, ctx->stream, ctx->ws_active, &ctx->prof);
 }
 
+// The same product with the roles of the operands exchanged, C^T = B^T A^T:
+// rows and columns of C swap, and with them the operands, their maps, batch
+// strides, storage flags and row-scale bounds (the k scale stays on k).
+GemmDesc transposed(const GemmDesc& d) {
+  GemmDesc t = d;
+  t.M = d.N;
+  t.N = d.M;
+  t.A = d.B;
+  t.Am = d.Bn;
+  t.Ak = d.Bk;
+  t.Ab = d.Bb;
+  t.B = d.A;
+  t.Bk = d.Ak;
+  t.Bn = d.Am;
+  t.Bb = d.Ab;
+  t.Cm = d.Cn;
+  t.Cn = d.Cm;
+  t.a32 = d.b32;
+  t.b32 = d.a32;
+  t.hintA = d.hintB;
+  t.hAb = d.hBb;
+  t.hintB = d.hintA;
+  t.hBb = d.hAb;
+  return t;
+}
+
 // U^T Gamma for the projector P: Gamma's AO+RI rows rotated onto the r kept
 // eigenvectors U of the ABS overlap.  With S+ = U diag(1/s) U^T every
 // contraction over the ABS index runs over those r rows, S+ becoming the
@@ -1234,6 +1260,13 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         const size_t kt_n = std::max(size_t(MNo) * nj, MrN * size_t(np_max));
         double* Kt = ctx->buf("blk1", k32 ? (kt_n + 1) / 2 : kt_n);
         auto kt_at = [&](size_t off) { return advance(Kt, int64_t(off), k32); };
+        // With FP32 storage the K~ and pair-sum products run transposed (rows (beta, k), columns
+        // (m, j)): their output columns are then the contiguous ABS rows m, so each lane stores its
+        // four columns as one 16-byte access (and reads them so for the pair sums' beta).
+        const bool kt_tr = k32 && Mr % 4 == 0;
+        auto run_kt = [&](const GemmDesc& g, const char* stage) {
+          run_gemm(ctx, kt_tr ? transposed(g) : g, stage);
+        };
         // X2sP[A + DX p] = X2sT[A + DX (j + a k)] for pair p = (j, k) of the packed order
         double* X2sP = ctx->buf("x2sp", size_t(n_pairs) * DV);
         // X2sT[A + DX (j + a k)] = sX_A O2_X[(j, k), A]: the DF index contiguous, so the
@@ -1308,13 +1341,13 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                   g.B = phiY;
                   g.C = S;
                   if (cs) g.hintB = nrmP;
-                  run_gemm(ctx, g, "3e_ind_fock_kt");
+                  run_kt(g, "3e_ind_fock_kt");
                 }
                 g.N = N * (a - pb.j0);
                 g.B = phiY + size_t(N) * (c + pb.j0);
                 g.C = advance(S, int64_t(MrN) * c, k32);
                 if (cs) g.hintB = nrmP + size_t(N) * (c + pb.j0);
-                run_gemm(ctx, g, "3e_ind_fock_kt");
+                run_kt(g, "3e_ind_fock_kt");
                 if (nbeyond > 0) {
                   // S[., c + k - j0 | j] += K~[m, (beta, c+j) | k] for the active k past the block:
                   // rows (m, k), columns (beta, j), one launch
@@ -1337,7 +1370,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                     h.hintA = nrmT + size_t(Mr) * (pb.j0 + pb.nb);
                     h.hintB = nrmP + size_t(N) * (c + pb.j0);
                   }
-                  run_gemm(ctx, h, "3e_ind_fock_kt_pair");
+                  run_kt(h, "3e_ind_fock_kt_pair");
                 }
               }
               const PairBlock& last = pblk[pgrp[gi].second - 1];
@@ -1384,7 +1417,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
                 g.hintA = nrmT + size_t(Mr) * j0;
                 g.hintB = nrmP;
               }
-              run_gemm(ctx, g, "3e_ind_fock_kt");
+              run_kt(g, "3e_ind_fock_kt");
               GemmDesc h;  // L[(m, beta), A] += sum_{(k, j)} K~[(m, beta), k | j] X2sT[A, j + a k]
               h.M = int(MrN);
               h.N = An;

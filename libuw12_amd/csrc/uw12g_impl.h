@@ -188,6 +188,8 @@ inline void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)
 
 // GEMM launch in the context's current contraction precision (see uw12g.cpp).
 void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact = false);
+// C^T = B^T A^T: the same product with rows and columns (and operands) exchanged.
+GemmDesc transposed(const GemmDesc& d);
 
 template <class F>
 void timed(uw12g_context* ctx, const char* stage, const char* kernel, double flops, double bytes, F&& f) {

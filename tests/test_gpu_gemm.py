@@ -252,6 +252,30 @@ def test_split_forms_fp32_storage(layout, mode, store):
         assert np.array_equal(got, f32(got))  # C came back from FP32 storage
 
 
+@pytest.mark.parametrize('mode', [3, 5])
+@pytest.mark.parametrize('store', [0, 4])
+@pytest.mark.parametrize('N', [200, 204])
+def test_split_forms_row_major_output(mode, store, N):
+    """Split forms writing an output whose columns are contiguous (row-major C, so each lane's
+    four columns are one vector store; beta C read the same way), FP64 or FP32 output storage,
+    batched, with a partial last tile of columns."""
+    M, K, batch = 300, 150, 2
+    rng = np.random.default_rng(hash(('rowmajor', mode, store, N)) % 2**32)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    A = rng.standard_normal(M * K * batch)
+    B = rng.standard_normal(K * N * batch)
+    C = rng.standard_normal(M * N * batch)
+    C = f32(C) if store & 4 else C
+    args = (M, N, K, batch, 1.25, 0.5, A, M2(1), M2(M), M * K, B, M2(1), M2(K), K * N, C, M2(N), M2(1), M * N)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode + 8 * store)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, M2(1), M2(M), M * K, B, M2(1), M2(K), K * N, M2(N), M2(1),
+                          M * N)
+    bound = bound + (2.0 ** -24 if store & 4 else 1e-15) * np.abs(ref)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound), (err / bound).max()
+
+
 def test_fp32_storage_needs_the_split_forms():
     """FP32 operand storage is read only by the plane-tensor path: the FP64 and
     FP32 forms refuse such descriptors before launching anything."""
