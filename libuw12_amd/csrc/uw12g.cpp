@@ -1260,9 +1260,10 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         const size_t kt_n = std::max(size_t(MNo) * nj, MrN * size_t(np_max));
         double* Kt = ctx->buf("blk1", k32 ? (kt_n + 1) / 2 : kt_n);
         auto kt_at = [&](size_t off) { return advance(Kt, int64_t(off), k32); };
-        // With FP32 storage the K~ and pair-sum products run transposed (rows (beta, k), columns
-        // (m, j)): their output columns are then the contiguous ABS rows m, so each lane stores its
-        // four columns as one 16-byte access (and reads them so for the pair sums' beta).
+        // With FP32 storage the K~, pair-sum and L products run transposed (K~: rows (beta, k),
+        // columns (m, j); L: rows A, columns (m, beta)): their output columns are then the
+        // contiguous ABS rows m, so each lane stores its four columns as one vector access (and
+        // reads them so for beta).
         const bool kt_tr = k32 && Mr % 4 == 0;
         auto run_kt = [&](const GemmDesc& g, const char* stage) {
           run_gemm(ctx, kt_tr ? transposed(g) : g, stage);
@@ -1390,7 +1391,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.Cm = lin(1);
               h.Cn = lin(int64_t(MrN));
               h.hintA = hintL;
-              run_gemm(ctx, h, "3e_ind_fock_l");
+              run_kt(h, "3e_ind_fock_l");
               progress(ctx, "same-spin 3e Fock, spin %d pass %d, A %d-%d of %d, pair group %d of %d (j %d-%d)", s,
                        pass, A0, A0 + An, DX, int(gi) + 1, int(pgrp.size()), pblk[pgrp[gi].first].j0,
                        last.j0 + last.nb);
@@ -1434,7 +1435,7 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
               h.Cm = lin(1);
               h.Cn = lin(int64_t(MrN));
               h.hintA = hintL;
-              run_gemm(ctx, h, "3e_ind_fock_l");
+              run_kt(h, "3e_ind_fock_l");
               if (pass == 0 && A0 == 0) {
                 // R[A, alpha | j] = sum_{(m, i)} Psi_W[m, c+i, A] K~[m, (alpha, c+i) | j], with jf
                 // of the block's j as extra columns (alpha, j) of one product: wider than N alone,
