@@ -124,6 +124,9 @@ def roofline_from_profile(records):
                     frac=round(achieved / peak, 4), traffic=None,
                     launches=n, avg_launch_us=round(1e3 * ms / n, 2),
                     algorithmic_flops_per_launch=fl / n, share_of_gpu_time=round(ms / total_ms, 3))
+        if by > 0:
+            # the library's count of each launch's compulsory HBM bytes (operands once, output, partials)
+            roof['compulsory_bytes_per_launch'] = by / n
     else:
         achieved = by / (ms * 1e-3) / 1e9
         roof = dict(bound='hbm', kernel=dom, achieved=round(achieved, 1), peak=PEAK_HBM_GBS, unit='GB/s',
@@ -496,6 +499,8 @@ def main():
     roof['traffic'] = measured_traffic(roof['kernel'], args.config, args.form)
     if roof['traffic'] is not None:
         roof['traffic_source'] = f'{traffic_profile(args.config, args.form)} (rocprofv3 FETCH_SIZE + WRITE_SIZE)'
+        if roof.get('compulsory_bytes_per_launch'):
+            roof['traffic_over_compulsory'] = round(roof['traffic'] / roof['compulsory_bytes_per_launch'], 3)
     if args.profile_table:
         print(json.dumps(m['table'], indent=1), file=sys.stderr)
     cpu = None

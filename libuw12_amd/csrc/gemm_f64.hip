@@ -2059,7 +2059,12 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
       prof->end(rec, stream,
                 "gemm_planes_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + std::to_string(np) + "," +
                     (f16 ? "1" : "0") + ">",
-                2.0 * d.M * double(d.N) * klen * d.batch, 0.0);
+                2.0 * d.M * double(d.N) * klen * d.batch,
+                // compulsory HBM bytes: both operands' planes over this k range once, the output
+                // (read too for beta), the split-K partials written
+                2.0 * np * double(klen) * (double(d.M) * nbA + double(d.N) * nbB) +
+                    double(d.M) * d.N * d.batch * (d.c32 ? 4.0 : 8.0) * (q.d.beta != 0.0 && q.splits == 1 ? 2.0 : 1.0) +
+                    (q.splits > 1 ? 8.0 * q.splits * double(d.M) * d.N * d.batch : 0.0));
     if (q.splits > 1) {
       const int64_t MN = int64_t(d.M) * d.N;
       dim3 g(unsigned(std::min<int64_t>((MN + 255) / 256, 4096)), unsigned(d.batch));
