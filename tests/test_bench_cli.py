@@ -28,3 +28,19 @@ def test_roofline_picks_dominant_instantiation_and_its_peak():
     assert roof['launches'] == 2
     roof64, _ = bench.roofline_from_profile(recs[2:])
     assert roof64['peak'] == bench.PEAK_FP64_TFLOPS
+
+
+def test_roofline_peak_of_the_split_kernels():
+    """The plane GEMMs are rated against the BF16 / FP16 dense peak over their products per
+    multiply-add (three with two planes, six with three), and carry their compulsory bytes."""
+    sys.path.insert(0, ROOT)
+    import bench
+    recs = [('3e_ind_fock_kt|gemm_planes_kernel<128,128,2,1>', 10.0, 3.0e12, 4.0e9),
+            ('3e_ind_fock_l|gemm_planes_kernel<128,128,2,1>', 5.0, 1.5e12, 2.0e9)]
+    roof, _ = bench.roofline_from_profile(recs)
+    assert roof['kernel'] == 'gemm_planes_kernel<128,128,2,1>'
+    assert roof['peak'] == round(bench.PEAK_BF16_TFLOPS / 3, 1)
+    assert abs(roof['compulsory_bytes_per_launch'] - 3.0e9) < 1.0
+    roof6, _ = bench.roofline_from_profile([('x|gemm_planes_kernel<128,128,3,0>', 1.0, 1e11, 0.0)])
+    assert roof6['peak'] == round(bench.PEAK_BF16_TFLOPS / 6, 1)
+    assert 'compulsory_bytes_per_launch' not in roof6
