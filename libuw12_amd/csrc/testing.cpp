@@ -74,6 +74,7 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
     const int store = naive >> 3;
     naive &= 7;
     if (naive < 0 || naive > 5) throw Error(-1, "mode must be 0 .. 5 (plus 8 / 16 / 32 for FP32 storage)");
+    if (store && naive == 1) throw Error(-2, "the one-thread-per-output kernel reads FP64 storage only");
     UW12G_HIP(hipSetDevice(device));
     Dev a, b, c, k;
     if (store & 1) a.up32(A, nA);
