@@ -36,9 +36,10 @@ int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, do
 /* Average time in milliseconds of `reps` launches of the same GEMM on arrays
  * already in device memory (after one untimed launch), on a stream of its own;
  * prec 0 runs the FP64 matrix cores, 1 the FP32 form, 2 / 3 the BF16x3 /
- * BF16x6 split forms, 4 the FP16x3 split form.  nA, nB, nC and nks are
- * the element counts of the arrays: a descriptor that would reach past one of
- * them is refused (-2) before anything is launched. */
+ * BF16x6 split forms, 4 the FP16x3 split form; + 8 / 16 / 32: A / B / C
+ * hold FP32 values (split forms).  nA, nB, nC and nks are the element counts
+ * of the arrays: a descriptor that would reach past one of them is refused
+ * (-2) before anything is launched. */
 int uw12g_time_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
                     size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, size_t nB,
                     uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,

@@ -954,81 +954,128 @@ struct PlaneDesc {
   }
 };
 
-// One thread per unit, r fastest (the unit writes of a wave are contiguous).
-// F16: two half planes of the scaled values (NP = 2); else NP bfloat16 planes.
+// A workgroup per 128 rows (a lane per row, so the unit writes of a wave are
+// contiguous), qc consecutive units (k runs of 8) and one batch, two units at a
+// time; the row offset and scale are taken once.  Inside one period of the k
+// map the 8 values of a unit are evenly spaced (one offset per unit); a unit
+// that crosses a period or K, or has a k scale, takes one offset per k.
+// F16: two half planes of the scaled values (NP = 2), for FP32 values scaled
+// and split in FP32 (ldexp is exact, so the words equal those of the FP64
+// steps); else NP bfloat16 planes.
 template <int NP, bool F16>
-__global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s) {
-  const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
-  const int64_t plane = units;
-  for (int64_t u = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; u < units; u += int64_t(gridDim.x) * blockDim.x) {
-    const int r = int(u % s.Rp);
-    const int64_t qb = u / s.Rp;
-    const int q = int(qb % s.KQ), b = int(qb / s.KQ);
-    const bool rok = r < s.R;
-    const int64_t base = s.Xb * b + (rok ? s.Xr.off(r) : 0);
-    double v[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = 8 * q + e;
-      const bool ok = rok && k < s.K;
-      double x = s.at(ok ? base + s.Xk.off(k) : 0);
-      if (s.ks) x *= s.ks[ok ? s.Kb * b + s.Km.off(k) : 0];
-      v[e] = ok ? x : 0.0;
-    }
-    if constexpr (F16) {
-      const double sc = s.scale[int64_t(b) * s.Rp + r];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] *= sc;
-    }
+__global__ __launch_bounds__(256) void split_planes_kernel(PlaneDesc s, int qc) {
+  const int r = int(blockIdx.x) * 128 + int(threadIdx.x & 127);
+  const int b = int(blockIdx.z);
+  const int q1 = min(s.KQ, int(blockIdx.y + 1) * qc);
+  const bool rok = r < s.R;
+  const int64_t base = s.Xb * b + (rok ? s.Xr.off(uint32_t(r)) : 0);
+  const int64_t plane = int64_t(s.nb) * s.KQ * s.Rp;
+  const double sc = F16 ? s.scale[int64_t(b) * s.Rp + r] : 1.0;
+  int ex = 0;
+  if constexpr (F16) {
+    (void)frexp(sc, &ex);  // sc = 2^(ex - 1)
+    --ex;
+  }
+  const bool f32 = F16 && s.x32 && !s.ks;
+  const float* xf = reinterpret_cast<const float*>(s.X);
+  for (int q = int(blockIdx.y) * qc + int(threadIdx.x >> 7); q < q1; q += 2) {
+    const int k0 = 8 * q;
+    const uint32_t pq = fd_div(uint32_t(k0), s.Xk.fd);
+    const uint32_t i0 = uint32_t(k0) - pq * s.Xk.n0;
+    const bool even = rok && !s.ks && k0 + 8 <= s.K && i0 + 8 <= s.Xk.n0;
+    const int64_t kb = base + int64_t(i0) * s.Xk.s0 + int64_t(pq) * s.Xk.s1;
     uint32_t w[4][NP];
+    if (f32) {
+      float v[8];
+      if (even) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      if constexpr (F16) {
-        static_assert(NP == 2, "the FP16 form has two planes");
-        uint32_t t2[2];
-        split_pair_f16(v[2 * i], v[2 * i + 1], t2);
-        w[i][0] = t2[0];
-        w[i][1] = t2[1];
+        for (int e = 0; e < 8; ++e) v[e] = xf[kb + e * s.Xk.s0];
       } else {
-        split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int k = k0 + e;
+          v[e] = rok && k < s.K ? xf[base + s.Xk.off(uint32_t(k))] : 0.0f;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float y0 = __builtin_amdgcn_ldexpf(v[2 * i], ex), y1 = __builtin_amdgcn_ldexpf(v[2 * i + 1], ex);
+        const uint32_t h = pack_f16(y0, y1);
+        w[i][0] = h;
+        w[i][NP - 1] = pack_f16(y0 - float(f16_lo(h)), y1 - float(f16_hi(h)));
+      }
+    } else {
+      double v[8];
+      if (even) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = s.at(kb + e * s.Xk.s0);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int k = k0 + e;
+          const bool ok = rok && k < s.K;
+          double x = s.at(ok ? base + s.Xk.off(uint32_t(k)) : 0);
+          if (s.ks) x *= s.ks[ok ? s.Kb * b + s.Km.off(uint32_t(k)) : 0];
+          v[e] = ok ? x : 0.0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        if constexpr (F16) {
+          static_assert(NP == 2, "the FP16 form has two planes");
+          uint32_t t2[2];
+          split_pair_f16(v[2 * i] * sc, v[2 * i + 1] * sc, t2);
+          w[i][0] = t2[0];
+          w[i][NP - 1] = t2[1];
+        } else {
+          split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
+        }
       }
     }
+    const int64_t u = (int64_t(b) * s.KQ + q) * s.Rp + r;
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) s.out[pl * plane + u] = u32x4{w[0][pl], w[1][pl], w[2][pl], w[3][pl]};
   }
 }
 
 // The same units for operands with k contiguous in memory and the rows strided
-// (the per-unit kernel above would read 64 scattered 8-byte words per wave and
-// load): a workgroup stages a tile of 64 rows x 32 k through LDS, loading along
-// k (32 consecutive k of one row per 32 lanes) and writing the units along the
-// rows (64 consecutive units per wave and plane).
+// (the kernel above would read 64 scattered words per wave and load): a
+// workgroup per 64 rows, kc consecutive 32-k tiles and one batch stages each
+// tile through LDS, loading along k (32 consecutive k of one row per 32 lanes)
+// and writing the units along the rows (64 consecutive units per wave and
+// plane); row offsets and scales are taken once.
 template <int NP, bool F16>
-__global__ __launch_bounds__(256) void split_planes_kc_kernel(PlaneDesc s) {
+__global__ __launch_bounds__(256) void split_planes_kc_kernel(PlaneDesc s, int kc) {
   __shared__ double T[64][33];
   const int64_t plane = int64_t(s.nb) * s.KQ * s.Rp;
-  const int64_t rtiles = s.Rp / 64;  // Rp is a multiple of 128
-  const int qtiles = s.KQ / 4;       // KQ = 4 ceil(K / 32)
-  const int64_t tiles = int64_t(s.nb) * qtiles * rtiles;
-  for (int64_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-    const int r0 = int(tile % rtiles) * 64;
-    const int64_t rest = tile / rtiles;
-    const int kt = int(rest % qtiles), b = int(rest / qtiles);
+  const int r0 = int(blockIdx.x) * 64, b = int(blockIdx.z);
+  const int lr = int(threadIdx.x >> 5), kk = int(threadIdx.x & 31);
+  int64_t ro[8];
+  double sc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int r = r0 + lr + 8 * i;
+    ro[i] = r < s.R ? s.Xb * b + s.Xr.off(uint32_t(r)) : -1;
+    sc[i] = F16 ? s.scale[int64_t(b) * s.Rp + r] : 1.0;
+  }
+  const int kt1 = min(s.KQ / 4, int(blockIdx.y + 1) * kc);  // KQ = 4 ceil(K / 32)
+  const int row = int(threadIdx.x & 63), qq = int(threadIdx.x >> 6);
+  for (int kt = int(blockIdx.y) * kc; kt < kt1; ++kt) {
+    const int k = 32 * kt + kk;
+    const bool kok = k < s.K;
+    const int64_t ko = kok ? s.Xk.off(uint32_t(k)) : 0;
+    const double ksv = s.ks && kok ? s.ks[s.Kb * b + s.Km.off(uint32_t(k))] : 1.0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int idx = int(threadIdx.x) + 256 * i;
-      const int row = idx >> 5, kk = idx & 31;
-      const int r = r0 + row, k = 32 * kt + kk;
       double x = 0.0;
-      if (r < s.R && k < s.K) {
-        x = s.at(s.Xb * b + s.Xr.off(r) + s.Xk.off(k));
-        if (s.ks) x *= s.ks[s.Kb * b + s.Km.off(k)];
-        if constexpr (F16) x *= s.scale[int64_t(b) * s.Rp + r];
+      if (kok && ro[i] >= 0) {
+        x = s.at(ro[i] + ko);
+        if (s.ks) x *= ksv;
+        if constexpr (F16) x *= sc[i];
       }
-      T[row][kk] = x;
+      T[lr + 8 * i][kk] = x;
     }
     __syncthreads();
-    const int row = int(threadIdx.x & 63), qq = int(threadIdx.x >> 6);
     double v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = T[row][8 * qq + e];
@@ -1039,7 +1086,7 @@ __global__ __launch_bounds__(256) void split_planes_kc_kernel(PlaneDesc s) {
         uint32_t t2[2];
         split_pair_f16(v[2 * i], v[2 * i + 1], t2);
         w[i][0] = t2[0];
-        w[i][1] = t2[1];
+        w[i][NP - 1] = t2[1];
       } else {
         split_pair<NP>(v[2 * i], v[2 * i + 1], w[i]);
       }
@@ -1770,17 +1817,19 @@ constexpr int planes_of(int prec) { return prec == 3 ? 3 : 2; }
 
 template <int NP, bool F16>
 void launch_split(const PlaneDesc& s, hipStream_t stream, Profiler* prof) {
-  const int64_t units = int64_t(s.nb) * s.KQ * s.Rp;
-  const unsigned grid = unsigned(std::min<int64_t>((units + 255) / 256, 8192));
   const bool timing = prof && prof->on;
   ProfileRec rec;
   if (timing) prof->begin(rec, stream);
+  // grid: row blocks, runs of k (at most 65535 of them), batches
   if (s.Xk.s0 == 1 && s.Xr.s0 != 1) {
-    const int64_t tiles = int64_t(s.nb) * (s.KQ / 4) * (s.Rp / 64);
-    const unsigned g = unsigned(std::min<int64_t>(tiles, 16384));
-    hipLaunchKernelGGL((split_planes_kc_kernel<NP, F16>), dim3(g), dim3(256), 0, stream, s);
+    const int tiles = s.KQ / 4;  // 32-k tiles
+    const int kc = std::max(8, (tiles + 65534) / 65535);
+    const dim3 g(unsigned(s.Rp / 64), unsigned((tiles + kc - 1) / kc), unsigned(s.nb));
+    hipLaunchKernelGGL((split_planes_kc_kernel<NP, F16>), g, dim3(256), 0, stream, s, kc);
   } else {
-    hipLaunchKernelGGL((split_planes_kernel<NP, F16>), dim3(grid), dim3(256), 0, stream, s);
+    const int qc = 2 * std::max(8, (s.KQ + 2 * 65535 - 1) / (2 * 65535));  // even
+    const dim3 g(unsigned(s.Rp / 128), unsigned((s.KQ + qc - 1) / qc), unsigned(s.nb));
+    hipLaunchKernelGGL((split_planes_kernel<NP, F16>), g, dim3(256), 0, stream, s, qc);
   }
   UW12G_HIP(hipGetLastError());
   if (timing)

@@ -134,8 +134,11 @@ extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int prec, int reps, double* ms) {
   try {
+    // bits 3..5: A, B, C hold FP32 values (split forms only); the counts are then of floats
+    const int store = prec >> 3;
+    prec &= 7;
     if (prec < 0 || prec > 4)
-      throw Error(-1, "prec must be 0 (FP64), 1 (FP32), 2 (BF16x3), 3 (BF16x6) or 4 (FP16x3)");
+      throw Error(-1, "prec must be 0 (FP64), 1 (FP32), 2 (BF16x3), 3 (BF16x6) or 4 (FP16x3), plus 8 / 16 / 32");
     if (M <= 0 || N <= 0 || K <= 0 || batch <= 0) throw Error(-1, "M, N, K and batch must be positive");
     // every element the descriptor addresses lies inside the caller's arrays (the
     // maps are checked non-negative, so the largest offset is the sum of the maxima)
@@ -173,6 +176,9 @@ extern "C" int uw12g_time_gemm(int device, int M, int N, int K, int batch, doubl
     d.Cb = Cb;
     d.ks = ks;
     d.Km = to_map(Km);
+    d.a32 = store & 1;
+    d.b32 = (store >> 1) & 1;
+    d.c32 = (store >> 2) & 1;
     struct Stream {
       hipStream_t s = nullptr;
       ~Stream() {
