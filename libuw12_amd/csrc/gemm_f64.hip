@@ -1174,11 +1174,15 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
   constexpr int WM = BM / 2, WN = BN / 2;
   constexpr int MB = WM / 16, NB = WN / 16;
   constexpr int UA = 4 * BM / NT, UB = 4 * BN / NT;  // units per thread, plane and slab
-  // two planes: two LDS buffers (80 KiB at 128 x 128, two workgroups per CU);
-  // three planes: one (its registers allow one set of staged loads)
+  // two planes: two LDS buffers for A (40 KiB at 128 x 128), and the B
+  // fragments loaded by each wave straight from the plane tensor (whose units
+  // are the fragments' 16-byte lane words) one slab ahead, so that only A goes
+  // through LDS; three planes: A and B through one LDS buffer (their registers
+  // allow one set of staged loads)
   constexpr int NBUF = NP == 2 ? 2 : 1;
+  constexpr bool BG = NP == 2;
   __shared__ __attribute__((aligned(16))) uint32_t As[NBUF][NP][BM][kRowWords];
-  __shared__ __attribute__((aligned(16))) uint32_t Bs[NBUF][NP][BN][kRowWords];
+  __shared__ __attribute__((aligned(16))) uint32_t Bs[BG ? 1 : NBUF][NP][BG ? 1 : BN][kRowWords];
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
@@ -1228,10 +1232,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
         const int u = t + NT * i;
         ra[pl][i] = Ab[pl * planeA + int64_t(qs + u / BM) * pa.RpA + u % BM];
       }
+      if constexpr (!BG) {
 #pragma unroll
-      for (int i = 0; i < UB; ++i) {
-        const int u = t + NT * i;
-        rb[pl][i] = Bb[pl * planeB + int64_t(qs + u / BN) * pa.RpB + u % BN];
+        for (int i = 0; i < UB; ++i) {
+          const int u = t + NT * i;
+          rb[pl][i] = Bb[pl * planeB + int64_t(qs + u / BN) * pa.RpB + u % BN];
+        }
       }
     }
   };
@@ -1243,10 +1249,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
         const int u = t + NT * i;
         *reinterpret_cast<u32x4*>(&As[buf()][pl][u % BM][4 * (u / BM)]) = ra[pl][i];
       }
+      if constexpr (!BG) {
 #pragma unroll
-      for (int i = 0; i < UB; ++i) {
-        const int u = t + NT * i;
-        *reinterpret_cast<u32x4*>(&Bs[buf()][pl][u % BN][4 * (u / BN)]) = rb[pl][i];
+        for (int i = 0; i < UB; ++i) {
+          const int u = t + NT * i;
+          *reinterpret_cast<u32x4*>(&Bs[buf()][pl][u % BN][4 * (u / BN)]) = rb[pl][i];
+        }
       }
     }
   };
@@ -1258,6 +1266,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
     for (int j = 0; j < NB; ++j) acc[i][j] = f4{0, 0, 0, 0};
   const int wm0 = (wave & 1) * WM, wn0 = (wave >> 1) * WN;
   const int lr = lane & 15, lk = lane >> 4;
+  // BG: this wave's B fragments of slab s (lane (lr, lk): unit lk of column wn0 + 16 j + lr)
+  auto ldB = [&](u32x4(&f)[NB][NP], int s) {
+    const int qs = q0 + 4 * min(s, nslab - 1) + lk;
+#pragma unroll
+    for (int pl = 0; pl < NP; ++pl)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) f[j][pl] = Bb[pl * planeB + int64_t(qs) * pa.RpB + wn0 + 16 * j + lr];
+  };
   // plane products (bfloat16 or half operands), smallest first: 3 products with
   // two planes, 6 with three; product p multiplies B plane kB[p] by A plane kA[p]
   constexpr int NPROD = NP == 3 ? 6 : 3;
@@ -1271,7 +1287,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
   // product first), then the products outermost, so that consecutive MFMAs go to
   // different accumulators (16 apart for 4 x 4 fragments) and never wait on each
   // other; each accumulator still sums its products in the same order.
-  auto compute = [&](auto buf) {
+  auto compute = [&](auto buf, const u32x4(&bg)[NB][NP]) {
     constexpr int kB[6] = {NP == 3 ? 2 : 1, 0, 1, 1, 0, 0};
     constexpr int kA[6] = {0, NP == 3 ? 2 : 1, 1, 0, 1, 0};
     constexpr int first = NP == 3 ? 0 : 3;  // two planes: the last three entries
@@ -1281,7 +1297,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
       afr[i][kA[first]] = *reinterpret_cast<const u32x4*>(&As[buf()][kA[first]][wm0 + 16 * i + lr][4 * lk]);
 #pragma unroll
     for (int j = 0; j < NB; ++j)
-      bfr[j][kB[first]] = *reinterpret_cast<const u32x4*>(&Bs[buf()][kB[first]][wn0 + 16 * j + lr][4 * lk]);
+      bfr[j][kB[first]] = BG ? bg[j][kB[first]]
+                             : *reinterpret_cast<const u32x4*>(&Bs[buf()][kB[first]][wn0 + 16 * j + lr][4 * lk]);
 #pragma unroll
     for (int pl = 0; pl < NP; ++pl) {
       if (pl != kA[first]) {
@@ -1292,7 +1309,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
       if (pl != kB[first]) {
 #pragma unroll
         for (int j = 0; j < NB; ++j)
-          bfr[j][pl] = *reinterpret_cast<const u32x4*>(&Bs[buf()][pl][wn0 + 16 * j + lr][4 * lk]);
+          bfr[j][pl] = BG ? bg[j][pl] : *reinterpret_cast<const u32x4*>(&Bs[buf()][pl][wn0 + 16 * j + lr][4 * lk]);
       }
     }
 #pragma unroll
@@ -1307,9 +1324,11 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
   // reads of the first product, then each MFMA paired with one of the other
   // fragment reads, LDS writes and global loads, so that they issue in the
   // shadow of the MFMAs.
-  constexpr int NRD0 = MB + NB, NRD1 = (MB + NB) * (NP - 1), NMF = MB * NB * NPROD;
-  constexpr int NWR = (UA + UB) * NP, NLD = (UA + UB) * NP;
+  // (BG: the B fragment loads first, LDS reads and writes of A only)
+  constexpr int NRD0 = BG ? MB : MB + NB, NRD1 = (BG ? MB : MB + NB) * (NP - 1), NMF = MB * NB * NPROD;
+  constexpr int NWR = (UA + (BG ? 0 : UB)) * NP, NLD = (UA + (BG ? 0 : UB)) * NP, NBL = BG ? NB * NP : 0;
   auto schedule = [&]() {
+    if constexpr (NBL > 0) __builtin_amdgcn_sched_group_barrier(0x020, NBL, 0);
     __builtin_amdgcn_sched_group_barrier(0x100, NRD0, 0);
 #pragma unroll
     for (int i = 0; i < NRD1; ++i) {
@@ -1339,8 +1358,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
       wr(xa, xb, B0{});
       __syncthreads();
       ld(xa, xb, 1);
+      u32x4 none[NB][NP];
       for (int s = 0;; ++s) {
-        compute(B0{});
+        compute(B0{}, none);
         if (s + 1 >= nslab) break;
         __syncthreads();
         wr(xa, xb, B0{});
@@ -1355,20 +1375,25 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
     // other register set holds the loads of slab s + 2 in flight.  One
     // barrier per slab.  Past the last slab the writes and loads repeat it
     // (ld clamps the slab), unread.
+    // The B fragments of slab s + 1 are loaded while slab s runs (bc, bn in turn).
     using B1 = std::integral_constant<int, NBUF - 1>;
+    u32x4 bc[NB][NP], bn[NB][NP];
     ld(xa, xb, 0);
+    ldB(bc, 0);
     ld(ya, yb, 1);
     wr(xa, xb, B0{});
     __syncthreads();
     ld(xa, xb, 2);
     for (int s = 0;; s += 2) {
-      compute(B0{});
+      ldB(bn, s + 1);
+      compute(B0{}, bc);
       wr(ya, yb, B1{});
       ld(ya, yb, s + 3);
       schedule();
       if (s + 1 >= nslab) break;
       __syncthreads();
-      compute(B1{});
+      ldB(bc, s + 2);
+      compute(B1{}, bn);
       wr(xa, xb, B0{});
       ld(xa, xb, s + 4);
       schedule();
