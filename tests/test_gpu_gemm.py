@@ -317,14 +317,15 @@ def test_fp16_form_rows_of_wide_range(layout):
 @pytest.mark.parametrize('mode', [0, 2, 3, 4, 5])
 @pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
 @pytest.mark.parametrize('kc', [False, True])
-@pytest.mark.parametrize('n0', [32, 16])
+@pytest.mark.parametrize('n0', [32, 16, 12])
 @pytest.mark.parametrize('bf16_path', ['planes', 'inkernel'], indirect=True)
 def test_affine_two_level_k_maps(mode, K, kc, n0, bf16_path):
     """Compound k = (r, q) with an inner period n0: with n0 a multiple of the slab
     depth (16 for FP64 / FP32, 32 for the BF16 forms) the unpredicated buffer-load
     path, in both the k-along-waves form (kc False) and the k-along-lanes form (kc
     True, where a k remainder runs as a second launch), with a k scale; with n0 = 16
-    the BF16 forms take the general path."""
+    the BF16 forms take the general path; with n0 = 12 some 8-k units of the plane
+    split cross a period (its per-k offsets)."""
     if n0 == 16 and mode < 3:
         pytest.skip('n0 = 16 is already slab-affine for the 16-deep slabs')
     if mode < 3 and bf16_path != 'planes':
@@ -390,3 +391,37 @@ def test_operand_span_past_lane_offsets(side, mode, bf16_path):
         bound = reduced_bound(mode, M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, Cm, Cn, 0)
         bound = np.pad(bound, (0, C.size - bound.size))  # elements past the output map are untouched
         assert np.all(np.abs(got - ref) <= bound + 1e-15 * np.abs(ref))
+
+
+@pytest.mark.parametrize('mode', [3, 5])
+@pytest.mark.parametrize('kc', [False, True])
+@pytest.mark.parametrize('n0', [12, 20])
+def test_split_forms_fp32_two_level_k(mode, kc, n0):
+    """FP32 storage of both operands read through two-level k maps whose period is not a multiple
+    of 8 (some 8-k units of the plane split cross it, and the last one passes K), for k strided with
+    the rows unit stride (kc False) and for k unit stride within the period with the rows strided
+    (kc True, the LDS-staged split): the same bound as FP64 storage on the rounded operands."""
+    rng = np.random.default_rng(hash(('f32k', mode, kc, n0)) % 2**32)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    M, N, K, batch = 170, 150, 157, 2
+    q = (K + n0 - 1) // n0
+    if kc:
+        Am, Ak = M2((n0 + 3) * q + 5), M2(1, n0 + 3, n0)
+        Bk, Bn = M2(1, n0 + 1, n0), M2((n0 + 1) * q + 2)
+    else:
+        Am, Ak = M2(1), M2(M + 3, (M + 3) * n0 + 7, n0)
+        Bk, Bn = M2(N + 1, (N + 1) * n0 + 5, n0), M2(1)
+    def span(m_r, m_k, R):
+        off = lambda m, i: i * m.s0 if m.n0 == 0 else (i % m.n0) * m.s0 + (i // m.n0) * m.s1
+        return int(off(m_r, np.arange(R)).max() + off(m_k, np.arange(K)).max()) + 1
+    sa, sb = span(Am, Ak, M), span(Bn, Bk, N)
+    A = f32(rng.standard_normal(sa * batch))
+    B = f32(rng.standard_normal(sb * batch))
+    C = rng.standard_normal(M * N * batch)
+    args = (M, N, K, batch, 0.75, 0.5, A, Am, Ak, sa, B, Bk, Bn, sb, C, M2(1), M2(M), M * N)
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode + 8 + 16)
+    bound = reduced_bound(mode, M, N, K, batch, 0.75, A, Am, Ak, sa, B, Bk, Bn, sb, M2(1), M2(M), M * N)
+    err = np.abs(got - ref)
+    assert np.all(err <= bound + 1e-15 * np.abs(ref)), (err / (bound + 1e-300)).max()
+    assert err.max() > 0
