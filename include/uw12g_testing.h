@@ -27,7 +27,9 @@ typedef struct {
  * form (operands rounded to FP32, FP32 sums per k chunk), 3 / 4 the BF16x3 /
  * BF16x6 split forms, 5 the FP16x3 split form; ks may be NULL.  With the split
  * forms, adding 8 / 16 / 32 to the mode keeps A / B / C in FP32 storage on the
- * device (the host values rounded to float; C is returned widened). */
+ * device (the host values rounded to float; C is returned widened), and adding
+ * 64 splits A once into a kept plane buffer that the product then reads (the
+ * path of a product whose A recurs). */
 int uw12g_test_gemm(int device, int M, int N, int K, int batch, double alpha, double beta, const double* A,
                     size_t nA, uw12g_map2 Am, uw12g_map2 Ak, long long Ab, const double* B, size_t nB,
                     uw12g_map2 Bk, uw12g_map2 Bn, long long Bb, double* C, size_t nC, uw12g_map2 Cm,

@@ -1928,18 +1928,72 @@ int64_t planes_slots(int kind) {
   return slots;
 }
 
+// Splits one operand into its plane tensor s.out; the FP16 form first sets the
+// rows' scales (scale, inv: nb Rp each) from the bounds `hint` when given, else
+// from a pass over the operand.
+void split_operand(PlaneDesc& s, int kind, const double* hint, int64_t hb, double* scale, double* inv,
+                   hipStream_t stream, Profiler* prof) {
+  if (kind == 4) {
+    if (hint) launch_hint_scale(s, hint, hb, scale, inv, stream);
+    else launch_row_scale(s, scale, inv, stream, prof);
+    s.scale = scale;
+    launch_split<2, true>(s, stream, prof);
+  } else if (planes_of(kind) == 2) {
+    launch_split<2, false>(s, stream, prof);
+  } else {
+    launch_split<3, false>(s, stream, prof);
+  }
+}
+
+// Plane-tensor layout of operand A of a split-form product: its batches, k
+// units, padded rows, plane bytes and (FP16) scale bytes.
+struct APlanes {
+  int nb = 1;
+  int64_t KQ = 0, Rp = 0, bytes = 0, scale_bytes = 0;
+};
+APlanes a_planes(const GemmDesc& d) {
+  APlanes a;
+  a.nb = (d.Ab == 0 && (!d.ks || d.Kb == 0)) ? 1 : d.batch;
+  a.KQ = 4 * ((int64_t(d.K) + BKB - 1) / BKB);
+  a.Rp = round_up(d.M, 128);
+  a.bytes = int64_t(planes_of(d.prec)) * a.nb * a.KQ * a.Rp * 16;
+  a.scale_bytes = d.prec == 4 ? 16 * int64_t(a.nb) * a.Rp : 0;
+  return a;
+}
+
+PlaneDesc a_desc(const GemmDesc& d, const APlanes& a, u32x4* out) {
+  PlaneDesc s;
+  s.X = d.A;
+  s.Xr = d.Am;
+  s.Xk = d.Ak;
+  s.Xb = d.Ab;
+  s.ks = d.ks;
+  s.Km = d.Km;
+  s.Kb = d.Kb;
+  s.R = d.M;
+  s.K = d.K;
+  s.nb = a.nb;
+  s.KQ = int(a.KQ);
+  s.Rp = int(a.Rp);
+  s.out = out;
+  s.x32 = d.a32;
+  return s;
+}
+
 // One product C = alpha A B + beta C of a split form over plane tensors, or
 // false when the workspace cannot hold its planes (then the caller runs the
-// in-kernel split of gemm_bf16x_kernel).  Large products run in pieces.
+// in-kernel split of gemm_bf16x_kernel).  Large products run in pieces.  With
+// d.planesA, A's planes (and FP16 scales) come from a split_a_planes buffer.
 bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   if (!ws || ws->carve) return false;
   const int kind = d.prec, np = planes_of(d.prec);
   const bool f16 = kind == 4;
-  const int nbA = (d.Ab == 0 && (!d.ks || d.Kb == 0)) ? 1 : d.batch;
+  const APlanes ap = a_planes(d);
+  const int nbA = ap.nb;
   const int nbB = d.Bb == 0 ? 1 : d.batch;
-  const int64_t KQ = 4 * ((int64_t(d.K) + BKB - 1) / BKB);
-  const int64_t RpA = round_up(d.M, 128), RpB = round_up(d.N, 128);
-  const int64_t bytesA = int64_t(np) * nbA * KQ * RpA * 16, bytesB = int64_t(np) * nbB * KQ * RpB * 16;
+  const int64_t KQ = ap.KQ;
+  const int64_t RpA = ap.Rp, RpB = round_up(d.N, 128);
+  const int64_t bytesA = d.planesA ? 0 : ap.bytes, bytesB = int64_t(np) * nbB * KQ * RpB * 16;
   if (KQ > 0x7fffffff / 8) return false;
 
   // tile shape as for the other forms
@@ -1997,6 +2051,11 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   const int64_t cap = g_plane_bytes_max > 0 ? g_plane_bytes_max : kPlaneBytesMax;
   const int64_t budget =
       std::min<int64_t>(cap, std::max<int64_t>(int64_t(ws->bytes), int64_t(ws->bytes) + int64_t(free_b) / 2));
+  if (need > budget && d.planesA) {  // pieces re-split A from its values
+    GemmDesc e = d;
+    e.planesA = nullptr;
+    return gemm_bf16_planes(e, stream, ws, prof);
+  }
   if (need > budget) {
     // pieces: batch halves, else row / column halves (whole periods of the maps)
     if (d.batch > 1) {
@@ -2049,6 +2108,11 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   double* partials = part ? reinterpret_cast<double*>(sbase + bytesS) : nullptr;
   double* scA = f16 ? reinterpret_cast<double*>(sbase) : nullptr;
   double* scB = f16 ? reinterpret_cast<double*>(sbase + round_up(16 * (int64_t(nbA) * RpA), 256)) : nullptr;
+  if (d.planesA) {  // A's planes, then its scales and inverse scales
+    const char* c = reinterpret_cast<const char*>(d.planesA);
+    pa.A = reinterpret_cast<const u32x4*>(c);
+    if (f16) scA = const_cast<double*>(reinterpret_cast<const double*>(c + round_up(ap.bytes, 256)));
+  }
   if (f16) {
     pa.invA = scA + int64_t(nbA) * RpA;
     pa.invB = scB + int64_t(nbB) * RpB;
@@ -2058,21 +2122,7 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   pa.RpA = int(RpA);
   pa.RpB = int(RpB);
   pa.KQ = int(KQ);
-  PlaneDesc sa;
-  sa.X = d.A;
-  sa.Xr = d.Am;
-  sa.Xk = d.Ak;
-  sa.Xb = d.Ab;
-  sa.ks = d.ks;
-  sa.Km = d.Km;
-  sa.Kb = d.Kb;
-  sa.R = d.M;
-  sa.K = d.K;
-  sa.nb = nbA;
-  sa.KQ = int(KQ);
-  sa.Rp = int(RpA);
-  sa.out = const_cast<u32x4*>(pa.A);
-  sa.x32 = d.a32;
+  PlaneDesc sa = a_desc(d, ap, const_cast<u32x4*>(pa.A));
   PlaneDesc sb;
   sb.X = d.B;
   sb.Xr = d.Bn;
@@ -2085,22 +2135,8 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   sb.Rp = int(RpB);
   sb.out = const_cast<u32x4*>(pa.B);
   sb.x32 = d.b32;
-  if (f16) {
-    if (d.hintA) launch_hint_scale(sa, d.hintA, d.hAb, scA, const_cast<double*>(pa.invA), stream);
-    else launch_row_scale(sa, scA, const_cast<double*>(pa.invA), stream, prof);
-    if (d.hintB) launch_hint_scale(sb, d.hintB, d.hBb, scB, const_cast<double*>(pa.invB), stream);
-    else launch_row_scale(sb, scB, const_cast<double*>(pa.invB), stream, prof);
-    sa.scale = scA;
-    sb.scale = scB;
-    launch_split<2, true>(sa, stream, prof);
-    launch_split<2, true>(sb, stream, prof);
-  } else if (np == 2) {
-    launch_split<2, false>(sa, stream, prof);
-    launch_split<2, false>(sb, stream, prof);
-  } else {
-    launch_split<3, false>(sa, stream, prof);
-    launch_split<3, false>(sb, stream, prof);
-  }
+  if (!d.planesA) split_operand(sa, kind, d.hintA, d.hAb, scA, const_cast<double*>(pa.invA), stream, prof);
+  split_operand(sb, kind, d.hintB, d.hBb, scB, const_cast<double*>(pa.invB), stream, prof);
 
   const int nlaunch = sequential ? nchunk : 1;
   for (int c = 0; c < nlaunch; ++c) {
@@ -2155,6 +2191,22 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
 }
 
 }  // namespace
+
+size_t plane_cache_bytes(const GemmDesc& d) {
+  if (d.prec < 2 || !g_bf16_planes || d.M <= 0 || d.K <= 0) return 0;
+  const APlanes a = a_planes(d);
+  if (a.KQ > 0x7fffffff / 8) return 0;
+  return size_t(round_up(a.bytes, 256) + a.scale_bytes);
+}
+
+void split_a_planes(const GemmDesc& d, void* out, hipStream_t stream, Profiler* prof) {
+  if (plane_cache_bytes(d) == 0) throw Error(-2, "split_a_planes: not a split-form product");
+  const APlanes a = a_planes(d);
+  char* c = reinterpret_cast<char*>(out);
+  double* scale = a.scale_bytes ? reinterpret_cast<double*>(c + round_up(a.bytes, 256)) : nullptr;
+  PlaneDesc s = a_desc(d, a, reinterpret_cast<u32x4*>(c));
+  split_operand(s, d.prec, d.hintA, d.hAb, scale, scale ? scale + int64_t(a.nb) * a.Rp : nullptr, stream, prof);
+}
 
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;

@@ -65,6 +65,8 @@ void row_norms(const double* x, int64_t rows, int64_t stride, int nt, double* ou
 // x column-major n0 x n1: out[i] = max_j x[i + n0 j] (over_cols) or out[j] = max_i x[i + n0 j].
 void max_over(const double* x, int64_t n0, int64_t n1, bool over_cols, double* out, hipStream_t s);
 // out[i + nu j] = c u[i] v[j].
+// out[i] = max_j |x[i + n0 j]| (a column-major n0 x n1 matrix, i < n0).
+void abs_max_rows(const double* x, int64_t n0, int64_t n1, double* out, hipStream_t s);
 void outer(const double* u, int64_t nu, const double* v, int64_t nv, double c, double* out, hipStream_t s);
 
 }  // namespace uw12g

@@ -562,6 +562,15 @@ __global__ void max_cols_kernel(const double* x, int64_t n0, int64_t n1, double*
   out[j] = m;
 }
 
+// Largest magnitude of each row of a column-major n0 x n1 matrix (out[i], i < n0).
+__global__ void abs_max_rows_kernel(const double* x, int64_t n0, int64_t n1, double* out) {
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i >= n0) return;
+  double m = 0.0;
+  for (int64_t j = 0; j < n1; ++j) m = fmax(m, fabs(x[i + n0 * j]));
+  out[i] = m;
+}
+
 __global__ void outer_kernel(const double* u, int64_t nu, const double* v, int64_t nv, double c, double* out) {
   const int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
   if (t >= nu * nv) return;
@@ -581,6 +590,12 @@ void max_over(const double* x, int64_t n0, int64_t n1, bool over_cols, double* o
     hipLaunchKernelGGL(max_rows_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, n0, n1, out);
   else
     hipLaunchKernelGGL(max_cols_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, s, x, n0, n1, out);
+  UW12G_HIP(hipGetLastError());
+}
+
+void abs_max_rows(const double* x, int64_t n0, int64_t n1, double* out, hipStream_t s) {
+  if (n0 <= 0) return;
+  hipLaunchKernelGGL(abs_max_rows_kernel, dim3(unsigned((n0 + 255) / 256)), dim3(256), 0, s, x, n0, n1, out);
   UW12G_HIP(hipGetLastError());
 }
 

@@ -70,13 +70,17 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
                                size_t nC, uw12g_map2 Cm, uw12g_map2 Cn, long long Cb, const double* ks, size_t nks,
                                uw12g_map2 Km, int naive) {
   try {
-    // mode bits 3..5: A, B, C held in FP32 storage on the device (split forms only)
-    const int store = naive >> 3;
+    // mode bits 3..5: A, B, C held in FP32 storage on the device (split forms only); bit 6:
+    // A split once into a kept plane buffer (split_a_planes) that the product then reads
+    const int store = (naive >> 3) & 7;
+    const bool keep = (naive >> 6) & 1;
     naive &= 7;
-    if (naive < 0 || naive > 5) throw Error(-1, "mode must be 0 .. 5 (plus 8 / 16 / 32 for FP32 storage)");
+    if (naive < 0 || naive > 5)
+      throw Error(-1, "mode must be 0 .. 5 (plus 8 / 16 / 32 for FP32 storage, 64 for kept A planes)");
     if (store && naive == 1) throw Error(-2, "the one-thread-per-output kernel reads FP64 storage only");
+    if (keep && naive < 3) throw Error(-2, "kept A planes: split forms only");
     UW12G_HIP(hipSetDevice(device));
-    Dev a, b, c, k;
+    Dev a, b, c, k, kept;  // kept: A's plane buffer (mode bit 6), freed after the synchronisation below
     if (store & 1) a.up32(A, nA);
     else a.up(A, nA);
     if (store & 2) b.up32(B, nB);
@@ -113,6 +117,13 @@ extern "C" int uw12g_test_gemm(int device, int M, int N, int K, int batch, doubl
       d.a32 = store & 1;
       d.b32 = (store >> 1) & 1;
       d.c32 = (store >> 2) & 1;
+      if (keep) {
+        const size_t bytes = plane_cache_bytes(d);
+        if (bytes == 0) throw Error(-2, "kept A planes: the product does not run over plane tensors");
+        kept.up(nullptr, (bytes + 7) / 8);
+        split_a_planes(d, kept.p, nullptr);
+        d.planesA = kept.p;
+      }
       gemm_f64(d, nullptr, &ws);
     }
     UW12G_HIP(hipDeviceSynchronize());

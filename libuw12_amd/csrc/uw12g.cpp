@@ -191,10 +191,7 @@ namespace uw12g {
 // and those that accumulate into the Fock matrix (long sums of terms that
 // cancel).  The mixed form runs the rest -- the large intermediate tensors of
 // the Fock terms (K~, L, R, Xi, G, X_i / Y_i) -- on FP32 matrix cores.
-void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact) {
-  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
-  ctx->prof.stage = stage;
-  GemmDesc e = d;
+int stage_prec(uw12g_context* ctx, const char* stage, bool exact) {
   const bool mixed = ctx->prec >= UW12G_MIXED;
   if (mixed && !exact)
     for (const std::string& x : ctx->exact_extra)
@@ -204,8 +201,56 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
                       : ctx->prec == UW12G_MIXED_BF16X6 ? 3
                       : ctx->prec == UW12G_MIXED_FP16X3 ? 4
                                                         : 1;
-  e.prec = mixed ? (exact ? 0 : reduced) : ctx->prec;
+  return mixed ? (exact ? 0 : reduced) : ctx->prec;
+}
+
+void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact) {
+  if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
+  ctx->prof.stage = stage;
+  GemmDesc e = d;
+  e.prec = stage_prec(ctx, stage, exact);
+  if (e.prec < 2) e.planesA = nullptr;
   gemm_f64(e, ctx->stream, ctx->ws_active, &ctx->prof);
+}
+
+// Product d of `stage` whose A operand is X's rotated Gamma, in a form over plane tensors: A's
+// planes are made on first use and kept on X across evaluations (d.planesA), or, when they would
+// take more than a quarter of the free device memory, the FP16 form keeps the largest magnitude
+// of each row of A (d.hintA: the same row scales without a pass over A).  Nothing in the
+// allocation-free mode.
+void use_kept_gt(uw12g_context* ctx, const uw12g_base_integrals* X, GemmDesc& d, const char* stage) {
+  if (ctx->arena || X->gt_id == 0 || d.A != X->gt.p || d.ks || d.batch != 1) return;
+  GemmDesc e = d;
+  e.prec = stage_prec(ctx, stage);
+  const size_t bytes = plane_cache_bytes(e);
+  if (bytes == 0) return;
+  if (X->gtp_id == X->gt_id && X->gtp_prec == e.prec && X->gtp.n * 8 >= bytes) {
+    d.planesA = X->gtp.p;
+    return;
+  }
+  size_t free_b = 0, total_b = 0;
+  UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (bytes <= X->gtp.n * 8 || bytes <= free_b / 4) {
+    X->gtp_id = 0;
+    if (X->gtp.p && bytes > X->gtp.n * 8) UW12G_HIP(hipStreamSynchronize(ctx->stream));
+    double* buf = X->gtp.get((bytes + 7) / 8);
+    ctx->prof.stage = stage;
+    split_a_planes(e, buf, ctx->stream, &ctx->prof);
+    X->gtp_id = X->gt_id;
+    X->gtp_prec = e.prec;
+    d.planesA = buf;
+    return;
+  }
+  // FP16 form, A(m, k) at m + M k (a column-major M x K matrix): its row maxima
+  if (e.prec != 4 || d.Am.n0 != 0xffffffffu || d.Am.s0 != 1 || d.Ak.n0 != 0xffffffffu || d.Ak.s0 != d.M) return;
+  if (X->gtr_id != X->gt_id || X->gtr.n < size_t(d.M)) {
+    X->gtr_id = 0;
+    if (X->gtr.p && size_t(d.M) > X->gtr.n) UW12G_HIP(hipStreamSynchronize(ctx->stream));
+    abs_max_rows(d.A, d.M, d.K, X->gtr.get(size_t(d.M)), ctx->stream);
+    X->gtr_id = X->gt_id;
+  }
+  d.hintA = X->gtr.p;
+  d.hAb = 0;
 }
 
 // The same product with the roles of the operands exchanged, C^T = B^T A^T:
@@ -231,6 +276,7 @@ GemmDesc transposed(const GemmDesc& d) {
   t.hAb = d.hBb;
   t.hintB = d.hintA;
   t.hBb = d.hAb;
+  t.planesA = nullptr;  // A's kept planes do not describe the new A
   return t;
 }
 
@@ -963,6 +1009,8 @@ void evaluate(uw12g_context* ctx, const uw12g_base_integrals* W, const uw12g_bas
         a1.C = Xi;
         a1.Cm = lin(1);
         a1.Cn = lin(int64_t(MrN));
+        // Gt_V is the A operand of every block and every evaluation: its planes (or row maxima) are kept
+        use_kept_gt(ctx, V, a1, "3e_ttdx_xi");
         run_gemm(ctx, a1, "3e_ttdx_xi");
         GemmDesc a3;  // F -= sum_{m, A} Gt_W[m, alpha, A0+A] (1/s_m) Xi[m, beta, A]
         a3.M = N;

@@ -159,6 +159,14 @@ struct uw12g_base_integrals {
   // projector whose id is gt_id, replaced when another projector comes)
   mutable DBuf gt;
   mutable uint64_t gt_id = 0;
+  // gt as the A operand of the t~ dX product Xi, kept across evaluations for the gt of projector
+  // id gtp_id / gtr_id: split into plane tensors (split_a_planes) for GEMM precision gtp_prec, or,
+  // when those would not fit, the largest magnitude of each of its rows (the FP16 row scales' bound)
+  mutable DBuf gtp;
+  mutable uint64_t gtp_id = 0;
+  mutable int gtp_prec = -1;
+  mutable DBuf gtr;
+  mutable uint64_t gtr_id = 0;
   // Non-stored integrals (uw12g_setup_base_integrals_gen with stored == 0):
   // no gamma; the DF metric's kept eigenvectors T[B, A] = Q[B, A] / sqrt|lambda_A|
   // (Nd x D) and the device generator, called again by every evaluation.
@@ -188,6 +196,8 @@ inline void set_device(uw12g_context* ctx) { UW12G_HIP(hipSetDevice(ctx->device)
 
 // GEMM launch in the context's current contraction precision (see uw12g.cpp).
 void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exact = false);
+// The GEMM arithmetic run_gemm uses for a stage (GemmDesc::prec).
+int stage_prec(uw12g_context* ctx, const char* stage, bool exact = false);
 // C^T = B^T A^T: the same product with rows and columns (and operands) exchanged.
 GemmDesc transposed(const GemmDesc& d);
 

@@ -100,6 +100,9 @@ struct GemmDesc {
   // split forms over plane tensors only: A / B / C hold FP32 values (float storage, maps in
   // elements); the other paths refuse such descriptors
   int a32 = 0, b32 = 0, c32 = 0;
+  // split forms over plane tensors only (optional): A already split by split_a_planes for a
+  // descriptor with the same A, maps, M, K, k scale, hint and prec; its split is then skipped
+  const void* planesA = nullptr;
 };
 
 // Allocation registry: every device buffer the library allocates is recorded
@@ -138,6 +141,12 @@ struct Profiler {
 
 struct Workspace;
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof = nullptr);
+// Operand A of a split-form product split once and kept (its planes, and for the FP16 form its
+// row scales), for a product that recurs with the same A: plane_cache_bytes is the size of that
+// buffer (0 when the product does not run over plane tensors), split_a_planes fills it, and the
+// descriptor's planesA then points at it.
+size_t plane_cache_bytes(const GemmDesc& d);
+void split_a_planes(const GemmDesc& d, void* out, hipStream_t stream, Profiler* prof = nullptr);
 // Test hook: a nonzero {BM, BN} forces that tile shape for every launch in the process.
 extern int g_force_tile[2];
 // Test hooks: g_bf16_planes 0 makes the BF16 split forms split their operands

@@ -425,3 +425,40 @@ def test_split_forms_fp32_two_level_k(mode, kc, n0):
     err = np.abs(got - ref)
     assert np.all(err <= bound + 1e-15 * np.abs(ref)), (err / (bound + 1e-300)).max()
     assert err.max() > 0
+
+
+@pytest.mark.parametrize('mode', [3, 4, 5])
+@pytest.mark.parametrize('case', ['plain', 'kscale_batched', 'fp32_a'])
+def test_kept_a_planes_match(mode, case):
+    """A split once into a kept plane buffer (the path of a product whose A recurs, such as the
+    rotated integrals of the t~ dX term) gives the same bits as splitting A within the product:
+    column-major A, a k-scaled batched A with two-level maps, and A in FP32 storage."""
+    rng = np.random.default_rng(hash(('kept', mode, case)) % 2**32)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    M, N, K, batch = 300, 170, 150, 1
+    Am, Ak, Ab, ks, Km, store = M2(1), M2(M), 0, None, None, 0
+    if case == 'kscale_batched':
+        batch = 2
+        Am, Ak, Ab = M2(1, 2 * M, 150), M2(150), 0
+        ks, Km = rng.standard_normal(K + 3), M2(1)
+    if case == 'fp32_a':
+        store = 1
+    A = rng.standard_normal(M * K * 2 + 2 * M * 2)
+    if store:
+        A = f32(A)
+    B = rng.standard_normal(K * N * batch)
+    C = rng.standard_normal(M * N * batch)
+    args = (M, N, K, batch, 1.25, 0.5, A, Am, Ak, Ab, B, M2(1), M2(K), K * N, C, M2(1), M2(M), M * N, ks, Km)
+    ref = run(*args, naive=mode + 8 * store)
+    got = run(*args, naive=mode + 8 * store + 64)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    exact = host_gemm(*args)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, Am, Ak, Ab, B, M2(1), M2(K), K * N, M2(1), M2(M), M * N,
+                          ks, Km)
+    assert np.all(np.abs(got - exact) <= bound + 1e-15 * np.abs(exact))
+
+
+def test_kept_a_planes_need_a_split_form():
+    A, B, C = np.ones(64), np.ones(64), np.zeros(64)
+    with pytest.raises(_lib.UW12Error):
+        run(8, 8, 8, 1, 1.0, 0.0, A, M2(1), M2(8), 0, B, M2(1), M2(8), 0, C, M2(1), M2(8), 0, naive=2 + 64)
