@@ -2208,6 +2208,42 @@ void split_a_planes(const GemmDesc& d, void* out, hipStream_t stream, Profiler* 
   split_operand(s, d.prec, d.hintA, d.hAb, scale, scale ? scale + int64_t(a.nb) * a.Rp : nullptr, stream, prof);
 }
 
+namespace {
+// FP64 / FP32 products whose k maps share an inner period n0 that is not a multiple of the slab
+// depth (the (m, A) contractions over the ABS rank r) would take the predicated general path:
+// the first n0 - n0 % bk of every period run as one product with slab-affine maps, the rest of
+// each period as a second product accumulated onto it.  False when that split does not apply.
+bool split_period_remainder(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
+  const int bk = slab_depth(d.prec);
+  const uint32_t n0 = d.Ak.n0;
+  if (d.prec >= 2 || n0 == 0xffffffffu || n0 % bk == 0 || n0 < uint32_t(bk) || d.Bk.n0 != n0 ||
+      (d.ks && d.Km.n0 != n0) || d.K % int64_t(n0) != 0)
+    return false;
+  const int q = int(d.K / int64_t(n0));
+  const uint32_t n1 = n0 - n0 % bk, r = n0 % bk;
+  GemmDesc m = d;
+  m.K = int(n1) * q;
+  m.Ak = two(n1, d.Ak.s0, d.Ak.s1);
+  m.Bk = two(n1, d.Bk.s0, d.Bk.s1);
+  if (d.ks) m.Km = two(n1, d.Km.s0, d.Km.s1);
+  if (!fast_maps_ok(m)) return false;
+  GemmDesc t = d;
+  t.K = int(r) * q;
+  t.beta = 1.0;
+  t.A = d.A + int64_t(n1) * d.Ak.s0;
+  t.B = d.B + int64_t(n1) * d.Bk.s0;
+  t.Ak = two(r, d.Ak.s0, d.Ak.s1);
+  t.Bk = two(r, d.Bk.s0, d.Bk.s1);
+  if (d.ks) {
+    t.ks = d.ks + int64_t(n1) * d.Km.s0;
+    t.Km = two(r, d.Km.s0, d.Km.s1);
+  }
+  gemm_f64(m, stream, ws, prof);
+  gemm_f64(t, stream, ws, prof);
+  return true;
+}
+}  // namespace
+
 void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   if (d.M <= 0 || d.N <= 0 || d.batch <= 0) return;
   check_desc(d);
@@ -2232,6 +2268,7 @@ void gemm_f64(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* pr
     gemm_f64(e, stream, ws, prof);
     return;
   }
+  if (!fast_maps_ok(d) && split_period_remainder(d, stream, ws, prof)) return;
   // coalesce along k only when that index has the unit stride and the other does not
   const bool akc = unit(d.Ak) && !unit(d.Am);
   const bool bkc = unit(d.Bk) && !unit(d.Bn);

@@ -317,7 +317,7 @@ def test_fp16_form_rows_of_wide_range(layout):
 @pytest.mark.parametrize('mode', [0, 2, 3, 4, 5])
 @pytest.mark.parametrize('K', [16 * 40, 16 * 40 + 7])
 @pytest.mark.parametrize('kc', [False, True])
-@pytest.mark.parametrize('n0', [32, 16, 12])
+@pytest.mark.parametrize('n0', [32, 16, 12, 20])
 @pytest.mark.parametrize('bf16_path', ['planes', 'inkernel'], indirect=True)
 def test_affine_two_level_k_maps(mode, K, kc, n0, bf16_path):
     """Compound k = (r, q) with an inner period n0: with n0 a multiple of the slab
@@ -325,7 +325,9 @@ def test_affine_two_level_k_maps(mode, K, kc, n0, bf16_path):
     path, in both the k-along-waves form (kc False) and the k-along-lanes form (kc
     True, where a k remainder runs as a second launch), with a k scale; with n0 = 16
     the BF16 forms take the general path; with n0 = 12 some 8-k units of the plane
-    split cross a period (its per-k offsets)."""
+    split cross a period (its per-k offsets); with n0 = 20 and whole periods (K = 640)
+    the FP64 / FP32 forms run the first 16 of every period on the unpredicated path
+    and the other 4 as a second product."""
     if n0 == 16 and mode < 3:
         pytest.skip('n0 = 16 is already slab-affine for the 16-deep slabs')
     if mode < 3 and bf16_path != 'planes':
