@@ -2210,33 +2210,42 @@ void split_a_planes(const GemmDesc& d, void* out, hipStream_t stream, Profiler* 
 
 namespace {
 // FP64 / FP32 products whose k maps share an inner period n0 that is not a multiple of the slab
-// depth (the (m, A) contractions over the ABS rank r) would take the predicated general path:
+// depth (the (m, A) contractions over the ABS rank r, k scales repeating every o occupied
+// orbitals) would take the predicated general path:
 // the first n0 - n0 % bk of every period run as one product with slab-affine maps, the rest of
 // each period as a second product accumulated onto it.  False when that split does not apply.
 bool split_period_remainder(const GemmDesc& d, hipStream_t stream, Workspace* ws, Profiler* prof) {
   const int bk = slab_depth(d.prec);
-  const uint32_t n0 = d.Ak.n0;
-  if (d.prec >= 2 || n0 == 0xffffffffu || n0 % bk == 0 || n0 < uint32_t(bk) || d.Bk.n0 != n0 ||
-      (d.ks && d.Km.n0 != n0) || d.K % int64_t(n0) != 0)
-    return false;
+  if (d.prec >= 2) return false;
+  // the common period of the two-level k maps (a one-level map of stride s is the two-level map
+  // (n0; s, n0 s) for any n0)
+  const uint32_t one = 0xffffffffu;
+  uint32_t n0 = one;
+  for (const Map2* k : {&d.Ak, &d.Bk, d.ks ? &d.Km : nullptr}) {
+    if (!k || k->n0 == one) continue;
+    if (n0 != one && k->n0 != n0) return false;
+    n0 = k->n0;
+  }
+  if (n0 == one || n0 % bk == 0 || n0 < uint32_t(bk) || d.K % int64_t(n0) != 0) return false;
+  auto s1 = [&](const Map2& k) { return k.n0 == one ? int64_t(n0) * k.s0 : k.s1; };
   const int q = int(d.K / int64_t(n0));
   const uint32_t n1 = n0 - n0 % bk, r = n0 % bk;
   GemmDesc m = d;
   m.K = int(n1) * q;
-  m.Ak = two(n1, d.Ak.s0, d.Ak.s1);
-  m.Bk = two(n1, d.Bk.s0, d.Bk.s1);
-  if (d.ks) m.Km = two(n1, d.Km.s0, d.Km.s1);
+  m.Ak = two(n1, d.Ak.s0, s1(d.Ak));
+  m.Bk = two(n1, d.Bk.s0, s1(d.Bk));
+  if (d.ks) m.Km = two(n1, d.Km.s0, s1(d.Km));
   if (!fast_maps_ok(m)) return false;
   GemmDesc t = d;
   t.K = int(r) * q;
   t.beta = 1.0;
   t.A = d.A + int64_t(n1) * d.Ak.s0;
   t.B = d.B + int64_t(n1) * d.Bk.s0;
-  t.Ak = two(r, d.Ak.s0, d.Ak.s1);
-  t.Bk = two(r, d.Bk.s0, d.Bk.s1);
+  t.Ak = two(r, d.Ak.s0, s1(d.Ak));
+  t.Bk = two(r, d.Bk.s0, s1(d.Bk));
   if (d.ks) {
     t.ks = d.ks + int64_t(n1) * d.Km.s0;
-    t.Km = two(r, d.Km.s0, d.Km.s1);
+    t.Km = two(r, d.Km.s0, s1(d.Km));
   }
   gemm_f64(m, stream, ws, prof);
   gemm_f64(t, stream, ws, prof);

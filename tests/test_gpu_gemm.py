@@ -464,3 +464,27 @@ def test_kept_a_planes_need_a_split_form():
     A, B, C = np.ones(64), np.ones(64), np.zeros(64)
     with pytest.raises(_lib.UW12Error):
         run(8, 8, 8, 1, 1.0, 0.0, A, M2(1), M2(8), 0, B, M2(1), M2(8), 0, C, M2(1), M2(8), 0, naive=2 + 64)
+
+
+@pytest.mark.parametrize('mode', [0, 2])
+@pytest.mark.parametrize('layout', ['NN', 'TT'])
+def test_k_scale_period_with_one_level_maps(mode, layout):
+    """A k scale repeating with a period (20) that is not a multiple of the slab depth while A and B
+    have one-level k maps (the direct Fock sum over (i, A) with weights per occupied i): the product
+    runs the first 16 of every period on the unpredicated path and the rest as a second product."""
+    rng = np.random.default_rng(hash(('kper', mode, layout)) % 2**32)
+    M, N, K = 150, 130, 20 * 33
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    A = rng.standard_normal(M * K)
+    B = rng.standard_normal(K * N)
+    C = rng.standard_normal(M * N)
+    ks = rng.standard_normal(20)
+    args = (M, N, K, 1, 0.75, 0.5, A, Am, Ak, 0, B, Bk, Bn, 0, C, M2(1), M2(M), 0, ks, M2(1, 0, 20))
+    ref = host_gemm(*args)
+    got = run(*args, naive=mode)
+    if mode == 0:
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    else:
+        bound = fp32_bound(M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, M2(1, 0, 20))
+        assert np.all(np.abs(got - ref) <= bound + 1e-15 * np.abs(ref))
