@@ -79,6 +79,12 @@ int uw12g_test_force_tile(int bm, int bn);
  * products run in pieces; 0 restores the default cap (16 GiB). */
 int uw12g_test_set_bf16_planes(int on, long long max_bytes);
 
+/* What the split forms keep of the rotated V integrals for the t~ dX product,
+ * for every later evaluation in the process: 1 (the default) its plane tensors,
+ * or its row maxima when those do not fit; 2 its row maxima only; 0 nothing
+ * (every product splits it again).  All give the same results (tests). */
+int uw12g_test_set_kept_gt(int mode);
+
 /* Host evaluation of the two-level map (checks the fast division). */
 long long uw12g_test_map2(uw12g_map2 m, unsigned i);
 

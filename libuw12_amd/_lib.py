@@ -29,7 +29,7 @@ EXPORTS = (
     'uw12g_context_profile',
     'uw12g_test_gemm', 'uw12g_test_map2', 'uw12g_time_gemm', 'uw12g_test_set_stream_cap',
     'uw12g_test_set_exact_stages', 'uw12g_test_set_block_cap', 'uw12g_test_set_tape_cap', 'uw12g_test_force_tile',
-    'uw12g_test_set_bf16_planes',
+    'uw12g_test_set_bf16_planes', 'uw12g_test_set_kept_gt',
 )
 
 
@@ -100,6 +100,7 @@ def load():
     L.uw12g_test_set_tape_cap.argtypes = [vp, ctypes.c_size_t]
     L.uw12g_test_force_tile.argtypes = [ctypes.c_int, ctypes.c_int]
     L.uw12g_test_set_bf16_planes.argtypes = [ctypes.c_int, ctypes.c_longlong]
+    L.uw12g_test_set_kept_gt.argtypes = [ctypes.c_int]
     L.uw12g_test_map2.argtypes = [Map2, ctypes.c_uint]
     L.uw12g_test_map2.restype = ctypes.c_longlong
     _lib = L

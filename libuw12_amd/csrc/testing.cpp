@@ -51,6 +51,15 @@ extern "C" int uw12g_test_set_bf16_planes(int on, long long max_bytes) {
   return 0;
 }
 
+extern "C" int uw12g_test_set_kept_gt(int mode) {
+  if (mode < 0 || mode > 2) {
+    set_last_error("mode must be 0, 1 or 2");
+    return -1;
+  }
+  g_kept_gt = mode;
+  return 0;
+}
+
 extern "C" long long uw12g_test_map2(uw12g_map2 m, unsigned i) { return (long long)to_map(m).off(i); }
 
 extern "C" int uw12g_test_force_tile(int bm, int bn) {

@@ -218,19 +218,21 @@ void run_gemm(uw12g_context* ctx, const GemmDesc& d, const char* stage, bool exa
 // take more than a quarter of the free device memory, the FP16 form keeps the largest magnitude
 // of each row of A (d.hintA: the same row scales without a pass over A).  Nothing in the
 // allocation-free mode.
+int g_kept_gt = 1;
+
 void use_kept_gt(uw12g_context* ctx, const uw12g_base_integrals* X, GemmDesc& d, const char* stage) {
-  if (ctx->arena || X->gt_id == 0 || d.A != X->gt.p || d.ks || d.batch != 1) return;
+  if (g_kept_gt == 0 || ctx->arena || X->gt_id == 0 || d.A != X->gt.p || d.ks || d.batch != 1) return;
   GemmDesc e = d;
   e.prec = stage_prec(ctx, stage);
   const size_t bytes = plane_cache_bytes(e);
   if (bytes == 0) return;
-  if (X->gtp_id == X->gt_id && X->gtp_prec == e.prec && X->gtp.n * 8 >= bytes) {
+  if (g_kept_gt == 1 && X->gtp_id == X->gt_id && X->gtp_prec == e.prec && X->gtp.n * 8 >= bytes) {
     d.planesA = X->gtp.p;
     return;
   }
   size_t free_b = 0, total_b = 0;
   UW12G_HIP(hipMemGetInfo(&free_b, &total_b));
-  if (bytes <= X->gtp.n * 8 || bytes <= free_b / 4) {
+  if (g_kept_gt == 1 && (bytes <= X->gtp.n * 8 || bytes <= free_b / 4)) {
     X->gtp_id = 0;
     if (X->gtp.p && bytes > X->gtp.n * 8) UW12G_HIP(hipStreamSynchronize(ctx->stream));
     double* buf = X->gtp.get((bytes + 7) / 8);

@@ -154,6 +154,9 @@ extern int g_force_tile[2];
 // g_plane_bytes_max > 0 caps the plane workspace of one product (bytes).
 extern int g_bf16_planes;
 extern int64_t g_plane_bytes_max;
+// Test hook: what the split forms keep of the rotated V integrals for the t~ dX product
+// (1 planes or row maxima, 2 row maxima only, 0 nothing).
+extern int g_kept_gt;
 void gemm_f64_naive(const GemmDesc& d, hipStream_t stream);
 
 // Device scratch owned by a context: grown on demand, reused across calls.

@@ -70,3 +70,36 @@ def test_scf_loop_on_kept_state_matches_fresh_handles(form):
         assert fock_error(F, F64) <= 1e-6
     del W, V, WV, P, P2
     ctx.close()
+
+
+@pytest.mark.parametrize('form', [u.MIXED_FP16X3, u.MIXED_BF16X3])
+def test_kept_gamma_modes_give_the_same_bits(form):
+    """The state kept for the t~ dX product (the rotated V integrals' plane tensors, or for the FP16
+    form only their row maxima when the planes would not fit) gives the same bits as splitting them
+    within every product, over two evaluations with moving orbitals."""
+    from libuw12_amd import _lib
+    L = _lib.load()
+    s = synth.make_system('m12u')
+    rng = np.random.default_rng(11)
+    N = s['N']
+    kw = dict(indirect_term=True, scale_opp_spin=1.0, scale_same_spin=0.5, precision=form)
+    orbs = [[np.array(c) for c in s['orbitals']]]
+    orbs.append([moved_orbitals(c, s['S'][:N, :N], rng, 0.05) for c in orbs[0]])
+    results = {}
+    try:
+        for mode in (0, 1, 2):
+            print(f'case m12u kept-gamma mode {mode} form {form}', flush=True)
+            L.uw12g_test_set_kept_gt(mode)
+            ctx = u.Context(0)
+            W, V, WV, P = gpu_setup(s, ctx)
+            results[mode] = [u.form_fock(W, V, WV, P, o, s['occ'], s['n_active'], **kw) for o in orbs]
+            del W, V, WV, P
+            ctx.close()
+    finally:
+        L.uw12g_test_set_kept_gt(1)
+    for mode in (1, 2):
+        for (e, F), (e0, F0) in zip(results[mode], results[0]):
+            for k in TERMS + ('total',):
+                assert e[k] == e0[k], (mode, k)
+            for a, b in zip(F, F0):
+                assert np.array_equal(a, b), (mode, np.abs(a - b).max())
