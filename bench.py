@@ -109,8 +109,9 @@ def roofline_from_profile(records):
     if fl > 0:
         achieved = fl / (ms * 1e-3) / 1e12
         # gemm_f64_kernel<BM, BN, AKC, BKC, KS, PREC>: PREC 1 runs on the FP32 matrix cores.
-        # gemm_bf16x_kernel<..., planes> and gemm_planes_kernel<BM, BN, planes, half>: 3 (two
-        # planes) or 6 (three planes) BF16 / FP16 products per multiply-add, so the peak of the
+        # gemm_bf16x_kernel<..., planes>, gemm_planes_kernel<BM, BN, planes, half> and
+        # gemm_planes_wide_kernel<half> (two planes): 3 (two planes) or 6 (three planes) BF16 / FP16
+        # products per multiply-add, so the peak of the
         # algorithmic FLOP rate is the BF16 (= FP16) dense peak over that count.
         args_ = dom.rstrip('>').split('<', 1)[-1].split(',')
         last = args_[-1]
@@ -118,6 +119,8 @@ def roofline_from_profile(records):
             peak = round(PEAK_BF16_TFLOPS / (3 if last == '2' else 6), 1)
         elif dom.startswith('gemm_planes_kernel<'):
             peak = round(PEAK_BF16_TFLOPS / (3 if args_[2] == '2' else 6), 1)
+        elif dom.startswith('gemm_planes_wide_kernel<'):  # two planes only
+            peak = round(PEAK_BF16_TFLOPS / 3, 1)
         else:
             peak = PEAK_FP32_TFLOPS if dom.startswith('gemm_f64_kernel<') and last == '1' else PEAK_FP64_TFLOPS
         roof = dict(bound='mfma', kernel=dom, achieved=round(achieved, 3), peak=peak, unit='TFLOP/s',

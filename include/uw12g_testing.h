@@ -68,8 +68,10 @@ int uw12g_test_set_tape_cap(uw12g_context* ctx, size_t doubles);
 int uw12g_test_set_exact_stages(uw12g_context* ctx, const char* prefixes);
 
 /* Force the GEMM tile shape (64 or 128 rows x 64 or 128 columns) for every
- * later launch in the process; 0, 0 restores the automatic choice (tests and
- * tile-choice measurements). */
+ * later launch in the process; 256, 256 forces the 256 x 256 tiles of the
+ * two-plane split forms (BF16x3, FP16x3) and leaves the other forms' choice
+ * automatic; 0, 0 restores the automatic choice (tests and tile-choice
+ * measurements). */
 int uw12g_test_force_tile(int bm, int bn);
 
 /* BF16 split forms: on != 0 (the default) splits the operands of each product

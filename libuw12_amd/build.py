@@ -15,7 +15,7 @@ BUILD_DIR = os.path.join(os.path.dirname(HERE), '_build', 'libuw12g')
 ROCM = os.environ.get('ROCM_PATH', '/opt/rocm')
 ARCH = os.environ.get('UW12G_ARCH', 'gfx950')
 
-SOURCES = ['gemm_f64.hip', 'kernels.hip', 'uw12g.cpp', 'uw12g_stream.cpp', 'testing.cpp']
+SOURCES = ['gemm_f64.hip', 'gemm_planes_wide.hip', 'kernels.hip', 'uw12g.cpp', 'uw12g_stream.cpp', 'testing.cpp']
 
 COMMON = [
     '-O3', '-std=c++17', '-fPIC', f'--offload-arch={ARCH}',

@@ -17,13 +17,13 @@
 #include <string>
 #include <type_traits>
 
+#include "gemm_planes.h"
 #include "kernels.h"
 #include "uw12g_internal.h"
 
 namespace uw12g {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
 
 int g_force_tile[2] = {0, 0};
 int g_bf16_planes = 1;
@@ -32,7 +32,6 @@ int64_t g_plane_bytes_max = 0;
 namespace {
 
 constexpr int BK = 16;
-constexpr int NT = 256;
 
 // LDS row padding (elements) of a tile loaded with k along the lanes: one
 // wave writes 16 k-rows of one or two tile columns at once.  FP64: an odd row
@@ -51,19 +50,6 @@ constexpr int kPadK64 = 1, kPadK32 = 18;
 // TF/s instead of 117.  Groups of 4, 8 and 16 rows were within 1% there; 4 was
 // 5% faster on the pair sums S (680 x 40 tiles).
 constexpr int kTileGroup = 4;
-
-struct Params {
-  GemmDesc d;
-  int tiles_m, tiles_n, splits, kchunk;
-  double* out;  // split-K partials (dense, M x N per split and batch) or nullptr
-  int64_t work;  // output tiles x k chunks x batches: one workgroup each
-  int64_t per;   // > 0: work items per XCD (contiguous ranges; the grid is 8 per)
-  int fast;     // 1: buffer-load main path (fast_ok); every slab of every chunk is full
-  int nfast;    // 1: tiles ordered with the column index fastest
-  int group;    // > 1 (with nfast 0): tiles ordered in groups of this many tile rows, row index fastest
-  int kbase;    // first k of chunk 0
-  int cvec = 0;  // plane GEMMs: C columns 4 n .. 4 n + 3 contiguous and 4-element aligned (vector epilogue)
-};
 
 // Buffer-resource loads.  The resource base is a wave-uniform address (scalar
 // registers); the per-lane part of an element address is a 32-bit byte offset
@@ -512,11 +498,8 @@ __global__ __launch_bounds__(NT) void gemm_f64_kernel(Params p) {
 // values of slab s + 1 (loaded during slab s - 1) into packed plane words in
 // registers; after a barrier the words go to LDS and the loads of slab s + 2
 // are issued.
-constexpr int BKB = 32;
 constexpr int kRowWords = BKB / 2 + 4;  // plane row stride in 32-bit words: 80 bytes, 16-byte aligned
 
-typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef float fl2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
 
@@ -557,7 +540,6 @@ __device__ inline void split_pair(double v0, double v1, uint32_t (&w)[NP]) {
 // by 1 / (s_row s_col) in the epilogue (exact).  Elements more than 2^18 below
 // their row's maximum lose low bits of l to the half subnormal range: their
 // absolute error stays below 2^-40 of the row maximum.
-typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h2 __attribute__((ext_vector_type(2)));
 
 __device__ inline uint32_t pack_f16(float a, float b) {
@@ -1156,14 +1138,6 @@ __global__ __launch_bounds__(256) void hint_scale_kernel(const double* hint, int
   inv[u] = iv;
 }
 
-struct PlaneArgs {
-  const u32x4* A = nullptr;
-  const u32x4* B = nullptr;
-  int nbA = 1, nbB = 1, RpA = 0, RpB = 0, KQ = 0;
-  const double* invA = nullptr;  // FP16 form: 1 / row scale of A (b RpA + m) and of B (b RpB + n)
-  const double* invB = nullptr;
-};
-
 // GEMM over plane tensors: the tile order, split-K partials and epilogue of
 // gemm_bf16x_kernel; k slabs of 32 loaded as 16-byte units (4 BM / 256 per
 // thread and plane) into registers two slabs ahead, written to the same LDS
@@ -1186,31 +1160,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
   const GemmDesc& d = p.d;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  int64_t w = blockIdx.x;
-  if (p.per > 0) {
-    w = int64_t(blockIdx.x & 7) * p.per + (blockIdx.x >> 3);
-    if (w >= p.work) return;
-  }
-  const int tiles = p.tiles_m * p.tiles_n;
-  const int tile = int(w % tiles);
-  const int64_t q = w / tiles;
-  const int split = int(q % p.splits);
-  const int b = int(q / p.splits);
-  int tm, tn;
-  if (p.nfast) {
-    tm = tile / p.tiles_n;
-    tn = tile % p.tiles_n;
-  } else if (p.group > 1) {
-    const int per_group = p.group * p.tiles_n;
-    const int g = tile / per_group, r = tile - g * per_group;
-    const int first = g * p.group;
-    const int gs = min(p.tiles_m - first, p.group);
-    tm = first + r % gs;
-    tn = r / gs;
-  } else {
-    tm = tile % p.tiles_m;
-    tn = tile / p.tiles_m;
-  }
+  int tm, tn, split, b;
+  if (!plane_tile(p, tm, tn, split, b)) return;
   const int m0 = tm * BM, n0 = tn * BN;
   const int k_begin = p.kbase + split * p.kchunk;
   const int k_end = min(d.K, k_begin + p.kchunk);
@@ -1402,90 +1353,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_planes_kernel(Params p, PlaneArgs 
     }
   }
 
-  auto ncol = [&](int j, int r) { return n0 + wn0 + 16 * j + 4 * lk + r; };
-  // FP16: alpha / (s_row s_col), a power of two times alpha
-  const double* invA = F16 ? pa.invA + int64_t(pa.nbA > 1 ? b : 0) * pa.RpA : nullptr;
-  const double* invB = F16 ? pa.invB + int64_t(pa.nbB > 1 ? b : 0) * pa.RpB : nullptr;
-  double colf[NB][4];
-#pragma unroll
-  for (int j = 0; j < NB; ++j)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int n = ncol(j, r);
-      colf[j][r] = F16 ? (n < d.N ? invB[n] : 0.0) * d.alpha : d.alpha;
-    }
-  if (p.out) {
-    double* W = p.out + (int64_t(split) * d.batch + b) * int64_t(d.M) * d.N;
-#pragma unroll
-    for (int i = 0; i < MB; ++i) {
-      const int m = m0 + wm0 + 16 * i + lr;
-      if (m >= d.M) continue;
-      const double rf = F16 ? invA[m] : 1.0;
-#pragma unroll
-      for (int j = 0; j < NB; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int n = ncol(j, r);
-          if (n < d.N) W[m + int64_t(d.M) * n] = colf[j][r] * rf * double(acc[i][j][r]);
-        }
-    }
-    return;
-  }
-  const int64_t cb = d.Cb * b;
-#pragma unroll
-  for (int i = 0; i < MB; ++i) {
-    const int m = m0 + wm0 + 16 * i + lr;
-    if (m >= d.M) continue;
-    const int64_t mo = d.Cm.off(m);
-    const double rf = F16 ? invA[m] : 1.0;
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      // a lane's four columns contiguous in C: one 16-byte access (FP32) or two (FP64)
-      if (p.cvec && ncol(j, 3) < d.N) {
-        const int64_t off = cb + mo + d.Cn.off(ncol(j, 0));
-        double v[4];
-#pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = colf[j][r] * rf * double(acc[i][j][r]);
-        if (d.c32) {
-          float4* c = reinterpret_cast<float4*>(reinterpret_cast<float*>(d.C) + off);
-          float4 o;
-          if (d.beta == 0.0) {
-            o = float4{float(v[0]), float(v[1]), float(v[2]), float(v[3])};
-          } else {
-            const float4 x = *c;
-            o = float4{float(v[0] + d.beta * double(x.x)), float(v[1] + d.beta * double(x.y)),
-                       float(v[2] + d.beta * double(x.z)), float(v[3] + d.beta * double(x.w))};
-          }
-          *c = o;
-        } else {
-          double2* c = reinterpret_cast<double2*>(d.C + off);
-          double2 o0{v[0], v[1]}, o1{v[2], v[3]};
-          if (d.beta != 0.0) {
-            const double2 x0 = c[0], x1 = c[1];
-            o0 = double2{v[0] + d.beta * x0.x, v[1] + d.beta * x0.y};
-            o1 = double2{v[2] + d.beta * x1.x, v[3] + d.beta * x1.y};
-          }
-          c[0] = o0;
-          c[1] = o1;
-        }
-        continue;
-      }
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int n = ncol(j, r);
-        if (n >= d.N) continue;
-        const int64_t off = cb + mo + d.Cn.off(n);
-        const double v = colf[j][r] * rf * double(acc[i][j][r]);
-        if (d.c32) {
-          float* c = reinterpret_cast<float*>(d.C) + off;
-          *c = float(d.beta == 0.0 ? v : v + d.beta * double(*c));
-        } else {
-          double* c = d.C + off;
-          *c = d.beta == 0.0 ? v : v + d.beta * *c;
-        }
-      }
-    }
-  }
+  planes_store<MB, NB, F16>(p, pa, acc, m0 + wm0, n0 + wn0, lr, lk, split, b);
 }
 
 // Sum split-K partials and apply beta / the output map (fixed order: deterministic).
@@ -1635,6 +1503,24 @@ int choose_splits(int64_t wgs, int kt, int K, int batch, int64_t slots) {
   return best_s;
 }
 
+// Whether the two-plane product of an M x N output (batch products, kt k
+// slabs of 32) runs faster on 256 x 256 tiles than on 128 x 128 tiles: each
+// choice's share of useful output (padding) times its occupancy of the
+// device over whole rounds of workgroups with the split-K count it would get,
+// the 256 x 256 tiles weighted by their speed on full rounds (1.1: 1.07 to
+// 1.14 measured on FP16x3 products from 25600 x 1280 x 960 to 8192^3).
+bool wide_pays(int M, int N, int K, int kt, int batch, int64_t slots128, int64_t slots256) {
+  auto score = [&](int t, int64_t slots) {
+    const int64_t tiles = int64_t((M + t - 1) / t) * ((N + t - 1) / t) * batch;
+    const int s = kt >= 16 ? choose_splits(tiles, kt, K, batch, slots) : 1;
+    const int64_t units = tiles * s;
+    const int64_t rounds = (units + slots - 1) / slots;
+    const double pad = double(M) * N / (double((M + t - 1) / t * t) * double((N + t - 1) / t * t));
+    return pad * double(units) / double(rounds * slots) / (1.0 + 60.0 * (2 * s + 1) / double(std::max(K, 1)));
+  };
+  return 1.1 * score(kWideTile, slots256) > score(128, slots128);
+}
+
 // Unit stride on the low level of a map (contiguous runs along that index).
 bool unit(const Map2& m) { return m.s0 == 1 && m.n0 >= 2; }
 
@@ -1740,7 +1626,8 @@ void launch_gemm(const GemmDesc& d, int kbase, bool fast, bool akc, bool bkc, hi
   double best = -1.0;
   for (int bm : {128, 64})
     for (int bn : {128, 64}) {
-      if (g_force_tile[0] && (bm != g_force_tile[0] || bn != g_force_tile[1])) continue;
+      if (g_force_tile[0] && g_force_tile[0] != kWideTile && (bm != g_force_tile[0] || bn != g_force_tile[1]))
+        continue;
       const double padded = double((d.M + bm - 1) / bm * bm) * double((d.N + bn - 1) / bn * bn);
       const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? (small ? 0.9 : 0.7)
                                                                                     : (small ? 0.95 : 0.9);
@@ -1955,7 +1842,7 @@ APlanes a_planes(const GemmDesc& d) {
   APlanes a;
   a.nb = (d.Ab == 0 && (!d.ks || d.Kb == 0)) ? 1 : d.batch;
   a.KQ = 4 * ((int64_t(d.K) + BKB - 1) / BKB);
-  a.Rp = round_up(d.M, 128);
+  a.Rp = round_up(d.M, kWideTile);
   a.bytes = int64_t(planes_of(d.prec)) * a.nb * a.KQ * a.Rp * 16;
   a.scale_bytes = d.prec == 4 ? 16 * int64_t(a.nb) * a.Rp : 0;
   return a;
@@ -1992,7 +1879,7 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   const int nbA = ap.nb;
   const int nbB = d.Bb == 0 ? 1 : d.batch;
   const int64_t KQ = ap.KQ;
-  const int64_t RpA = ap.Rp, RpB = round_up(d.N, 128);
+  const int64_t RpA = ap.Rp, RpB = round_up(d.N, kWideTile);
   const int64_t bytesA = d.planesA ? 0 : ap.bytes, bytesB = int64_t(np) * nbB * KQ * RpB * 16;
   if (KQ > 0x7fffffff / 8) return false;
 
@@ -2002,7 +1889,8 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
   double best = -1.0;
   for (int bm : {128, 64})
     for (int bn : {128, 64}) {
-      if (g_force_tile[0] && (bm != g_force_tile[0] || bn != g_force_tile[1])) continue;
+      if (g_force_tile[0] && g_force_tile[0] != kWideTile && (bm != g_force_tile[0] || bn != g_force_tile[1]))
+        continue;
       const double padded = double((d.M + bm - 1) / bm * bm) * double((d.N + bn - 1) / bn * bn);
       const double speed = (bm == 128 && bn == 128) ? 1.0 : (bm == 64 && bn == 64) ? (small ? 0.9 : 0.7)
                                                                                     : (small ? 0.95 : 0.9);
@@ -2013,6 +1901,12 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
         BN = bn;
       }
     }
+  // two planes: 256 x 256 tiles (gemm_planes_wide_kernel) where they pay
+  const bool wide = np == 2 && (g_force_tile[0] == kWideTile ||
+                                (!g_force_tile[0] && BM == 128 && BN == 128 &&
+                                 wide_pays(d.M, d.N, d.K, int(KQ / 4), d.batch, planes_slots<128, 128>(kind),
+                                           wide_planes_slots(kind))));
+  if (wide) BM = BN = kWideTile;
   Params p;
   p.d = d;
   p.tiles_m = (d.M + BM - 1) / BM;
@@ -2029,8 +1923,9 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
              reinterpret_cast<uintptr_t>(d.C) % 16 == 0 ? 1 : 0;
   }
   const int kt = int(KQ / 4);
-  const int64_t slots = BM == 128 ? (BN == 128 ? planes_slots<128, 128>(kind) : planes_slots<128, 64>(kind))
-                                  : (BN == 128 ? planes_slots<64, 128>(kind) : planes_slots<64, 64>(kind));
+  const int64_t slots = wide        ? wide_planes_slots(kind)
+                        : BM == 128 ? (BN == 128 ? planes_slots<128, 128>(kind) : planes_slots<128, 64>(kind))
+                                    : (BN == 128 ? planes_slots<64, 128>(kind) : planes_slots<64, 64>(kind));
   int splits = kt >= 16 ? choose_splits(tiles * d.batch, kt, d.K, d.batch, slots) : 1;
   // FP32 sums over at most kF32Chunk k values: that many chunks at least
   const int nchunk = (d.K + kF32Chunk - 1) / kF32Chunk;
@@ -2160,15 +2055,17 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
     const bool timing = prof && prof->on;
     ProfileRec rec;
     if (timing) prof->begin(rec, stream);
-    if (BM == 128 && BN == 128) launch_planes_cfg<128, 128>(q, pa, kind, grid, stream);
+    if (wide) launch_planes_wide(q, pa, kind, grid, stream);
+    else if (BM == 128 && BN == 128) launch_planes_cfg<128, 128>(q, pa, kind, grid, stream);
     else if (BM == 128) launch_planes_cfg<128, 64>(q, pa, kind, grid, stream);
     else if (BN == 128) launch_planes_cfg<64, 128>(q, pa, kind, grid, stream);
     else launch_planes_cfg<64, 64>(q, pa, kind, grid, stream);
     UW12G_HIP(hipGetLastError());
     if (timing)
       prof->end(rec, stream,
-                "gemm_planes_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," + std::to_string(np) + "," +
-                    (f16 ? "1" : "0") + ">",
+                (wide ? std::string("gemm_planes_wide_kernel<") + (f16 ? "1" : "0") + ">"
+                      : "gemm_planes_kernel<" + std::to_string(BM) + "," + std::to_string(BN) + "," +
+                            std::to_string(np) + "," + (f16 ? "1" : "0") + ">"),
                 2.0 * d.M * double(d.N) * klen * d.batch,
                 // compulsory HBM bytes: both operands' planes over this k range once, the output
                 // (read too for beta), the split-K partials written

@@ -63,9 +63,9 @@ extern "C" int uw12g_test_set_kept_gt(int mode) {
 extern "C" long long uw12g_test_map2(uw12g_map2 m, unsigned i) { return (long long)to_map(m).off(i); }
 
 extern "C" int uw12g_test_force_tile(int bm, int bn) {
-  const bool ok = (bm == 0 && bn == 0) || ((bm == 64 || bm == 128) && (bn == 64 || bn == 128));
+  const bool ok = (bm == 0 && bn == 0) || (bm == 256 && bn == 256) || ((bm == 64 || bm == 128) && (bn == 64 || bn == 128));
   if (!ok) {
-    set_last_error("tile must be 0 x 0 (automatic) or 64 / 128 in each dimension");
+    set_last_error("tile must be 0 x 0 (automatic), 256 x 256 or 64 / 128 in each dimension");
     return -1;
   }
   g_force_tile[0] = bm;

@@ -488,3 +488,71 @@ def test_k_scale_period_with_one_level_maps(mode, layout):
     else:
         bound = fp32_bound(M, N, K, 1, 0.75, A, Am, Ak, 0, B, Bk, Bn, 0, M2(1), M2(M), 0, ks, M2(1, 0, 20))
         assert np.all(np.abs(got - ref) <= bound + 1e-15 * np.abs(ref))
+
+
+@pytest.fixture
+def tiles():
+    """Sets the forced tile shape with tiles(bm, bn); restores the automatic choice afterwards."""
+    L = _lib.load()
+    yield lambda bm, bn: _lib.check(L.uw12g_test_force_tile(bm, bn))
+    L.uw12g_test_force_tile(0, 0)
+
+
+WIDE_CASES = [
+    # name, M, N, K, batch, layout
+    ('nn', 600, 520, 200, 1, 'NN'),
+    ('ragged_tn', 257, 300, 129, 1, 'TN'),
+    ('ragged_nt', 513, 250, 45, 1, 'NT'),
+    ('batched_tt', 300, 290, 150, 2, 'TT'),
+    ('small', 40, 33, 70, 3, 'NN'),
+]
+
+
+@pytest.mark.parametrize('name,M,N,K,batch,layout', WIDE_CASES)
+@pytest.mark.parametrize('mode', [3, 5])
+@pytest.mark.parametrize('extra', [0, 8 * 7, 64])
+def test_wide_tiles_match(tiles, name, M, N, K, batch, layout, mode, extra):
+    """The 256 x 256 tiles of the two-plane split forms (operands moved straight
+    into LDS, eight waves of 128 x 64) give the bits of the 128 x 128 tiles: ragged
+    edges, all layouts, batches, FP32 storage of A, B and C, and a kept A plane
+    buffer; K below 512 keeps both on one k chunk."""
+    rng = np.random.default_rng(hash(('wide', name, mode, extra)) % 2**32)
+    f32 = lambda x: x.astype(np.float32).astype(np.float64)
+    A = rng.standard_normal(M * K * batch)
+    B = rng.standard_normal(K * N * batch)
+    C = rng.standard_normal(M * N * batch)
+    if extra == 56:
+        A, B, C = f32(A), f32(B), f32(C)
+    Am, Ak = (M2(1), M2(M)) if layout[0] == 'N' else (M2(K), M2(1))
+    Bk, Bn = (M2(1), M2(K)) if layout[1] == 'N' else (M2(N), M2(1))
+    args = (M, N, K, batch, 1.25, 0.5, A, Am, Ak, M * K, B, Bk, Bn, K * N, C, M2(1), M2(M), M * N)
+    tiles(128, 128)
+    ref = run(*args, naive=mode + extra)
+    tiles(256, 256)
+    got = run(*args, naive=mode + extra)
+    assert np.array_equal(got, ref), np.abs(got - ref).max()
+    exact = host_gemm(*args)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, Am, Ak, M * K, B, Bk, Bn, K * N, M2(1), M2(M), M * N)
+    bound = bound + (2.0 ** -24 if extra == 56 else 1e-15) * np.abs(exact)
+    assert np.all(np.abs(got - exact) <= bound)
+
+
+@pytest.mark.parametrize('mode', [3, 5])
+@pytest.mark.parametrize('bf16_path', ['planes', 'pieces'], indirect=True)
+def test_wide_tiles_split_k_within_bound(tiles, mode, bf16_path):
+    """256 x 256 tiles with split-K partials and several FP32 k chunks (K = 5000 on few tiles),
+    whole or in pieces, within the form's rounding bound."""
+    rng = np.random.default_rng(hash(('wide_k', mode, bf16_path)) % 2**32)
+    M, N, K, batch = 520, 300, 5000, 1
+    A = rng.standard_normal(M * K)
+    B = rng.standard_normal(K * N)
+    C = rng.standard_normal(M * N)
+    args = (M, N, K, batch, 1.25, 0.5, A, M2(1), M2(M), 0, B, M2(N), M2(1), 0, C, M2(1), M2(M), 0)
+    tiles(256, 256)
+    got = run(*args, naive=mode)
+    exact = host_gemm(*args)
+    bound = reduced_bound(mode, M, N, K, batch, 1.25, A, M2(1), M2(M), 0, B, M2(N), M2(1), 0, M2(1), M2(M), 0)
+    assert np.all(np.abs(got - exact) <= bound + 1e-15 * np.abs(exact))
+    tiles(128, 128)
+    ref = run(*args, naive=mode)
+    assert np.abs(got - ref).max() <= 2 * bound.max()
