@@ -98,6 +98,16 @@ __global__ __launch_bounds__(2 * NT, 2) void gemm_planes_wide_kernel(Params p, P
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[i][j] = mma(bf[j][0], af[i][0], acc[i][j]);
+    // the first product's fragment reads, then each of the other reads paired
+    // with one of its MFMAs: every read is in flight before the first MFMA
+    // that needs it, instead of reads and waits spread through the slab
+    __builtin_amdgcn_sched_group_barrier(0x100, MB + NB, 0);
+#pragma unroll
+    for (int i = 0; i < MB + NB; ++i) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 3 * MB * NB - MB - NB, 0);
   };
 
   if (nslab > 0) issue(0, 0);
