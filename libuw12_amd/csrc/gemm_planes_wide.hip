@@ -77,7 +77,9 @@ __global__ __launch_bounds__(2 * NT, 2) void gemm_planes_wide_kernel(Params p, P
                                                      0, 0);
   };
   // slab s from buffer s % 2; the transfers of slab s + 1 are issued once the
-  // reads of the first product's fragments are in flight
+  // reads of the first product's fragments are in flight (the last slab
+  // repeats its own transfers into the idle buffer, unread, so that the slab
+  // has no branch; they are waited for before the epilogue)
   auto compute = [&](int s) {
     const int buf = s & 1;
     u32x4 af[MB][NP], bf[NB][NP];
@@ -85,7 +87,7 @@ __global__ __launch_bounds__(2 * NT, 2) void gemm_planes_wide_kernel(Params p, P
     for (int j = 0; j < NB; ++j) bf[j][1] = L[buf][1][1][lk][wn0 + 16 * j + lr];
 #pragma unroll
     for (int i = 0; i < MB; ++i) af[i][0] = L[buf][0][0][lk][wm0 + 16 * i + lr];
-    if (s + 1 < nslab) issue(s + 1, buf ^ 1);
+    issue(min(s + 1, nslab - 1), buf ^ 1);
 #pragma unroll
     for (int j = 0; j < NB; ++j) bf[j][0] = L[buf][1][0][lk][wn0 + 16 * j + lr];
 #pragma unroll
@@ -102,9 +104,12 @@ __global__ __launch_bounds__(2 * NT, 2) void gemm_planes_wide_kernel(Params p, P
     for (int i = 0; i < MB; ++i)
 #pragma unroll
       for (int j = 0; j < NB; ++j) acc[i][j] = mma(bf[j][0], af[i][0], acc[i][j]);
+    // the first product's fragment reads, the transfers of the next slab, then
     // each of the other fragment reads paired with one of the first product's
     // MFMAs: every read is in flight before the first MFMA that needs it,
     // instead of reads and waits spread through the slab
+    __builtin_amdgcn_sched_group_barrier(0x100, MB + NB, 0);
+    __builtin_amdgcn_sched_group_barrier(0x020, 4 * NP, 0);
 #pragma unroll
     for (int i = 0; i < MB + NB; ++i) {
       __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
@@ -121,6 +126,7 @@ __global__ __launch_bounds__(2 * NT, 2) void gemm_planes_wide_kernel(Params p, P
     __syncthreads();
     compute(s);
   }
+  __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0): no transfer into LDS outlives the workgroup
   planes_store<MB, NB, F16>(p, pa, acc, m0 + wm0, n0 + wn0, lr, lk, split, b);
 }
 
