@@ -77,41 +77,58 @@ __device__ __forceinline__ void planes_store(const Params& p, const PlaneArgs& p
     return;
   }
   const int64_t cb = d.Cb * b;
+  // a lane's four columns of fragment j contiguous in C: one 16-byte access (FP32) or two (FP64)
+  auto vec = [&](int j) { return p.cvec && ncol(j, 3) < d.N; };
 #pragma unroll
   for (int i = 0; i < MB; ++i) {
     const int m = m0w + 16 * i + lr;
     if (m >= d.M) continue;
     const int64_t mo = d.Cm.off(m);
     const double rf = F16 ? invA[m] : 1.0;
+    // beta != 0: the old values of the row group are all loaded before any of
+    // its stores, so that a wave waits on memory once per row group instead of
+    // once per fragment
+    double old[NB][4];
+    if (d.beta != 0.0) {
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (vec(j)) {
+          const int64_t off = cb + mo + d.Cn.off(ncol(j, 0));
+          if (d.c32) {
+            const float4 x = *reinterpret_cast<const float4*>(reinterpret_cast<const float*>(d.C) + off);
+            old[j][0] = x.x, old[j][1] = x.y, old[j][2] = x.z, old[j][3] = x.w;
+          } else {
+            const double2* c = reinterpret_cast<const double2*>(d.C + off);
+            const double2 x0 = c[0], x1 = c[1];
+            old[j][0] = x0.x, old[j][1] = x0.y, old[j][2] = x1.x, old[j][3] = x1.y;
+          }
+        } else {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int n = ncol(j, r);
+            const int64_t off = cb + mo + d.Cn.off(n < d.N ? n : 0);
+            old[j][r] = n >= d.N ? 0.0 : d.c32 ? double(reinterpret_cast<const float*>(d.C)[off]) : d.C[off];
+          }
+        }
+      }
+    }
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-      // a lane's four columns contiguous in C: one 16-byte access (FP32) or two (FP64)
-      if (p.cvec && ncol(j, 3) < d.N) {
-        const int64_t off = cb + mo + d.Cn.off(ncol(j, 0));
-        double v[4];
+      double v[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) v[r] = colf[j][r] * rf * double(acc[i][j][r]);
-        if (d.c32) {
-          float4* c = reinterpret_cast<float4*>(reinterpret_cast<float*>(d.C) + off);
-          float4 o;
-          if (d.beta == 0.0) {
-            o = float4{float(v[0]), float(v[1]), float(v[2]), float(v[3])};
-          } else {
-            const float4 x = *c;
-            o = float4{float(v[0] + d.beta * double(x.x)), float(v[1] + d.beta * double(x.y)),
-                       float(v[2] + d.beta * double(x.z)), float(v[3] + d.beta * double(x.w))};
-          }
-          *c = o;
-        } else {
+      for (int r = 0; r < 4; ++r) {
+        v[r] = colf[j][r] * rf * double(acc[i][j][r]);
+        if (d.beta != 0.0) v[r] = v[r] + d.beta * old[j][r];
+      }
+      if (vec(j)) {
+        const int64_t off = cb + mo + d.Cn.off(ncol(j, 0));
+        if (d.c32)
+          *reinterpret_cast<float4*>(reinterpret_cast<float*>(d.C) + off) =
+              float4{float(v[0]), float(v[1]), float(v[2]), float(v[3])};
+        else {
           double2* c = reinterpret_cast<double2*>(d.C + off);
-          double2 o0{v[0], v[1]}, o1{v[2], v[3]};
-          if (d.beta != 0.0) {
-            const double2 x0 = c[0], x1 = c[1];
-            o0 = double2{v[0] + d.beta * x0.x, v[1] + d.beta * x0.y};
-            o1 = double2{v[2] + d.beta * x1.x, v[3] + d.beta * x1.y};
-          }
-          c[0] = o0;
-          c[1] = o1;
+          c[0] = double2{v[0], v[1]};
+          c[1] = double2{v[2], v[3]};
         }
         continue;
       }
@@ -120,14 +137,8 @@ __device__ __forceinline__ void planes_store(const Params& p, const PlaneArgs& p
         const int n = ncol(j, r);
         if (n >= d.N) continue;
         const int64_t off = cb + mo + d.Cn.off(n);
-        const double v = colf[j][r] * rf * double(acc[i][j][r]);
-        if (d.c32) {
-          float* c = reinterpret_cast<float*>(d.C) + off;
-          *c = float(d.beta == 0.0 ? v : v + d.beta * double(*c));
-        } else {
-          double* c = d.C + off;
-          *c = d.beta == 0.0 ? v : v + d.beta * *c;
-        }
+        if (d.c32) reinterpret_cast<float*>(d.C)[off] = float(v[r]);
+        else d.C[off] = v[r];
       }
     }
   }
