@@ -1902,8 +1902,10 @@ bool gemm_bf16_planes(const GemmDesc& d, hipStream_t stream, Workspace* ws, Prof
       }
     }
   // two planes: 256 x 256 tiles (gemm_planes_wide_kernel) where they pay
+  // (beta != 0: the 128 x 128 tiles, whose two workgroups per CU overlap one's read-modify-write
+  // epilogue with the other's MFMAs; 20480 x 5120 x 960 with beta = 1: 0.95 against 1.00 ms)
   const bool wide = np == 2 && (g_force_tile[0] == kWideTile ||
-                                (!g_force_tile[0] && BM == 128 && BN == 128 &&
+                                (!g_force_tile[0] && BM == 128 && BN == 128 && d.beta == 0.0 &&
                                  wide_pays(d.M, d.N, d.K, int(KQ / 4), d.batch, planes_slots<128, 128>(kind),
                                            wide_planes_slots(kind))));
   if (wide) BM = BN = kWideTile;
