@@ -82,3 +82,32 @@ def test_c2_random_vs_oracle(gpu_ctx, fixture, cases):
     for key in KEYS:
         assert abs(eg[key] - eo[key]) <= 1e-12 * max(1.0, abs(eo[key])), (key, eg[key], eo[key])
     assert fock_error(Fg, [Fo]) <= 1e-10 * np.abs(Fo).max()
+
+
+def test_c2_fp16x3_on_either_plane_tile(gpu_ctx, cases):
+    """The FP16x3 form at full c2 with every two-plane product forced onto the 256 x 256 tiles, and
+    with none of them: both within the 1e-6 gate of the FP64 path, and within 1e-7 Eh and 1e-7 Fock
+    of each other (they differ only where the tile shape changes the split-K chunking)."""
+    print('case c2 recipe_k05 fp16x3 tiles', flush=True)
+    L = u._lib.load()
+    system, kappa = cases['recipe_k05']
+    W, V, WV, P = gpu_setup(system, gpu_ctx)
+    args = (W, V, WV, P, system['orbitals'], system['occ'], system['n_active'])
+    kw = dict(scale_opp_spin=kappa[0], scale_same_spin=kappa[1])
+    e64, F64 = u.form_fock(*args, **kw, precision=u.FP64)
+    out = {}
+    try:
+        for tile in ((256, 256), (128, 128)):
+            u._lib.check(L.uw12g_test_force_tile(*tile))
+            out[tile] = u.form_fock(*args, **kw, precision=u.MIXED_FP16X3)
+    finally:
+        L.uw12g_test_force_tile(0, 0)
+    for tile, (e, F) in out.items():
+        print(f'  tile {tile}: Fock {fock_error(F, F64):.2e}', flush=True)
+        for key in KEYS:
+            assert abs(e[key] - e64[key]) <= 1e-6, (tile, key, e[key], e64[key])
+        assert fock_error(F, F64) <= 1e-6
+    (ew, Fw), (en, Fn) = out[(256, 256)], out[(128, 128)]
+    for key in KEYS:
+        assert abs(ew[key] - en[key]) <= 1e-7, (key, ew[key], en[key])
+    assert fock_error(Fw, Fn) <= 1e-7
