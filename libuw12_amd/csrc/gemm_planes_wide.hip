@@ -5,9 +5,11 @@
 // per MFMA that the 128 x 128 tiles move from L2.  The units move from the
 // plane tensors straight into LDS with global_load_lds_dwordx4 (no staging
 // registers): a slab is 4 k units of 256 rows per operand and plane, in the
-// order of the plane tensors, and LDS holds two slabs (128 KiB).  The loads of
-// slab s + 1 are issued after the barrier that opens slab s, so they have one
-// slab of MFMAs (3 x 32 per wave) to land.  128 accumulators and 96 fragment
+// order of the plane tensors, and LDS holds two slabs (128 KiB).  One barrier
+// per slab; after it each wave reads the first product's fragments of slab s,
+// issues its transfers of slab s + 1 into the other buffer (they have one slab
+// of MFMAs, 3 x 32 per wave, to land), then reads its other fragments paired
+// with the first product's MFMAs.  128 accumulators and 96 fragment
 // registers per wave fit the 256 VGPRs of two waves per SIMD.  Per accumulator
 // the plane products and slabs run in the order of gemm_planes_kernel, and the
 // epilogue is the same: the results equal those of the 128 x 128 tiles bit for
