@@ -150,6 +150,9 @@ int64_t wide_planes_slots(int kind) {
 }
 
 void launch_planes_wide(const Params& p, const PlaneArgs& pa, int kind, dim3 grid, hipStream_t s) {
+  // every workgroup reads whole 256-row panels of both plane tensors
+  if (pa.RpA % kWideTile != 0 || pa.RpB % kWideTile != 0 || kind == 3)
+    throw Error(-2, "wide plane GEMM: plane rows not padded to 256, or three planes");
   if (kind == 2) hipLaunchKernelGGL((gemm_planes_wide_kernel<false>), grid, dim3(2 * NT), 0, s, p, pa);
   else hipLaunchKernelGGL((gemm_planes_wide_kernel<true>), grid, dim3(2 * NT), 0, s, p, pa);
 }
